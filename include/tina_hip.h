@@ -1,0 +1,205 @@
+/*
+ * tina_hip.h -- C ABI of the MI355X rasterise-and-shade library (libtina_hip.so).
+ *
+ * The reference (taichi_three / Tina) has no FFI: its "operator API" is a set of
+ * duck-typed Python protocols that Taichi inlines at JIT time.  Each entry point
+ * below replaces one Taichi kernel of that path; the comment on each names the
+ * reference function it stands in for (paths relative to the reference root).
+ *
+ * Conventions
+ *   - Every pointer named "d_*" is a device (HBM) pointer; everything else is host.
+ *   - Screen buffers are row-major [y][x] with y = 0 at the bottom row, i.e. the
+ *     transpose of Taichi's [x, y] fields (tina/core/engine.py:9-11).
+ *   - Triangle soups are [T][3][3] float32 (face, corner, xyz), the layout of
+ *     TriangleRaster.verts (tina/core/triangle.py:18).
+ *   - Every call returns TINA_OK (0) or an error code; tina_last_error() gives the
+ *     message.  No C++ exception crosses this boundary.  Calls only enqueue work on
+ *     the given stream (a hipStream_t passed as void*; NULL = default stream).
+ *   - A context is not thread-safe.  All device scratch is allocated by
+ *     tina_ctx_create / tina_ctx_resize, never inside the per-frame calls.
+ */
+#ifndef TINA_HIP_H
+#define TINA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define TINA_OK            0
+#define TINA_ERR_INVALID   1   /* bad argument (null pointer, bad size, bad enum) */
+#define TINA_ERR_CAPACITY  2   /* more faces / lights / ops than the context holds */
+#define TINA_ERR_HIP       3   /* a HIP runtime call failed */
+#define TINA_ERR_ALLOC     4   /* device allocation failed (or exceeds the set limit) */
+
+/* ---- raster flags (TriangleRaster options, tina/core/triangle.py:6-7) ---- */
+#define TINA_CULL   1u   /* drop faces with facing <= 0 (triangle.py:95-98) */
+#define TINA_CLIP   2u   /* drop faces whose 3 vertices are all outside NDC (triangle.py:100-104) */
+
+/* ---- shade flags --------------------------------------------------------- */
+#define TINA_SMOOTH   1u    /* interpolate per-vertex normals (triangle.py:36-38) */
+#define TINA_TEX      2u    /* interpolate per-vertex texcoords (triangle.py:43-46) */
+#define TINA_VCOLOR   4u    /* interpolate per-vertex colours for Input('color') (extension) */
+#define TINA_TONEMAP  8u    /* apply ACES to written pixels (tina/postp/tonemap.py:9-12) */
+#define TINA_BG      16u    /* write the background colour where no face won */
+#define TINA_IBL     32u    /* SkyboxLighting instead of Lighting (tina/core/lighting.py:4-21) */
+#define TINA_NOLIGHT 64u    /* skip direct lights (IBL only) */
+
+/* ---- limits -------------------------------------------------------------- */
+#define TINA_MAX_LIGHTS  16   /* Lighting(maxlights=16), tina/core/lighting.py:25 */
+#define TINA_MAX_PARAMS  12
+#define TINA_MAX_OPS     16
+#define TINA_MAX_STACK    4
+
+/* Camera: Engine.W2V / V2W / bias (tina/core/engine.py:14-17, 72-76).
+ * Matrices are row-major float32, M[i][j] = m[4*i + j]. */
+typedef struct tina_camera {
+    float W2V[16];
+    float V2W[16];
+    float bias[2];
+    float pad[2];
+} tina_camera;
+
+/* Lighting fields (tina/core/lighting.py:26-29): light_dirs vec4 (w = 0 for a
+ * directional light, 1 for a point light), light_colors vec3, ambient vec3. */
+typedef struct tina_lights {
+    int32_t n;
+    int32_t pad[3];
+    float dirs[TINA_MAX_LIGHTS][4];
+    float colors[TINA_MAX_LIGHTS][4];
+    float ambient[4];
+} tina_lights;
+
+/* Material parameter sources: the lowered form of tina.matr.nodes
+ * (tina/matr/nodes.py:42-111) plus FresnelFactor (tina/matr/material.py:68-83). */
+enum tina_param_kind {
+    TINA_P_CONST    = 0,  /* Const: value[0..2]                                  */
+    TINA_P_PARAM    = 1,  /* Param: device scalar/vec3 at ptr (dim 1 or 3)        */
+    TINA_P_COLOR    = 2,  /* Input('color'): per-vertex colour or (1,1,1)         */
+    TINA_P_POS      = 3,  /* Input('pos')                                         */
+    TINA_P_NORMAL   = 4,  /* Input('normal')                                      */
+    TINA_P_TEXCOORD = 5,  /* Input('texcoord') (z = 0)                            */
+    TINA_P_TEXTURE  = 6,  /* Texture: bilerp(tex[w][h][c], slot(a).xy * (shape-1)) */
+    TINA_P_FACE     = 7,  /* per-face array at ptr, dim 1 or 3 (extension)        */
+    TINA_P_FRESNEL  = 8   /* FresnelFactor(metallic=slot a, albedo=slot b, specular=slot c) */
+};
+
+typedef struct tina_param {
+    int32_t kind;
+    int32_t dim;          /* 1 (scalar, broadcast) or 3 */
+    int32_t a, b, c;      /* operand slots (FRESNEL, TEXTURE) */
+    int32_t tex_w, tex_h, tex_c;
+    float value[4];       /* CONST */
+    uint64_t ptr;         /* device pointer: PARAM / TEXTURE / FACE */
+} tina_param;
+
+/* Material program: post-order walk of the material tree (tina/matr/material.py).
+ * Leaves push a lobe value, combinators pop and push. */
+enum tina_op_kind {
+    TINA_OP_LAMBERT      = 0,  /* Lambert (material.py:387-442)                     */
+    TINA_OP_PHONG        = 1,  /* Phong(shineness=slot a) (material.py:445-457)     */
+    TINA_OP_BLINNPHONG   = 2,  /* BlinnPhong(shineness=slot a) (extension)          */
+    TINA_OP_COOKTORRANCE = 3,  /* CookTorrance(roughness=a, fresnel=b) (:244-362)   */
+    TINA_OP_EMISSION     = 4,  /* Emission (material.py:636-657)                    */
+    TINA_OP_SCALE        = 5,  /* ScaleMaterial(factor=slot a) (material.py:148-191) */
+    TINA_OP_MIX          = 6,  /* MixMaterial(factor=slot a) (material.py:86-145)   */
+    TINA_OP_ADD          = 7   /* AddMaterial (material.py:194-238)                 */
+};
+
+typedef struct tina_op {
+    int32_t op, a, b, pad;
+} tina_op;
+
+typedef struct tina_material {
+    int32_t nparams, nops;
+    int32_t pad[2];
+    tina_param params[TINA_MAX_PARAMS];
+    tina_op ops[TINA_MAX_OPS];
+} tina_material;
+
+/* Per-object face data bound by TriangleRaster.set_object (triangle.py:71-86).
+ * All pointers are device pointers; optional ones may be 0. */
+typedef struct tina_mesh {
+    uint64_t verts;    /* [T][3][3] f32 */
+    uint64_t norms;    /* [T][3][3] f32 (TINA_SMOOTH) */
+    uint64_t coors;    /* [T][3][2] f32 (TINA_TEX)    */
+    uint64_t colors;   /* [T][3][3] f32 (TINA_VCOLOR) */
+    int32_t nfaces;
+    uint32_t id_base;  /* global id of face 0: earlier objects own lower ids */
+} tina_mesh;
+
+/* Image-based lighting tables (SkyboxLighting, tina/core/lighting.py:4-21):
+ * equirect maps [w][h][3] f32 (x-major, indexed like Skybox.img[x, y],
+ * tina/skybox.py:39-145).  spec[0] is the environment itself (also the
+ * background); spec[1..6] are the prefiltered levels of CookTorrance.rough_levels.
+ * rough_lo/hi/step are the f32 values of (rough - step, rough, step) for the six
+ * levels, computed in f64 on the host as Python does (material.py:251-253). */
+#define TINA_IBL_LEVELS 7
+typedef struct tina_ibl {
+    uint64_t diff;                    /* Lambert irradiance map          */
+    int32_t diff_w, diff_h;
+    uint64_t spec[TINA_IBL_LEVELS];   /* env + 6 GGX-prefiltered levels  */
+    int32_t spec_w[TINA_IBL_LEVELS], spec_h[TINA_IBL_LEVELS];
+    float lut[3][3][2];               /* LUT texels [0..2]x[0..2] (x = VoH, y = roughness, [A,B]):
+                                         bilerp at coordinates in [0,1]^2 reads only these (material.py:275) */
+    float rough_lo[6], rough_hi[6], rough_step[6];
+} tina_ibl;
+
+typedef struct tina_ctx tina_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+
+/* Engine(res_x, res_y) + TriangleRaster(maxfaces) storage (engine.py:6-28,
+ * triangle.py:6-29).  Allocates the packed depth|id buffer and raster scratch. */
+int tina_ctx_create(int width, int height, int max_faces, tina_ctx **out);
+int tina_ctx_destroy(tina_ctx *ctx);
+/* Grows the face capacity (TriangleRaster maxfaces) without touching the keys. */
+int tina_ctx_reserve(tina_ctx *ctx, int max_faces);
+/* Message of the last failed call on ctx (or of the last failed create when ctx is NULL). */
+const char *tina_last_error(const tina_ctx *ctx);
+/* Library version, for the Python loader's sanity check. */
+int tina_version(void);
+
+/* ---- per-frame path ------------------------------------------------------ */
+
+/* Engine.set_camera / bias (engine.py:72-76, :17) -- host copy, passed by value. */
+int tina_set_camera(tina_ctx *ctx, const tina_camera *cam);
+
+/* Engine.clear_depth (engine.py:67-70) + TriangleRaster.occup = -1 (triangle.py:90-91):
+ * every packed key becomes (depth 2**30, no face). */
+int tina_clear(tina_ctx *ctx, void *stream);
+
+/* TriangleRaster.render_occup (triangle.py:88-131): setup, cull, clip, coverage,
+ * depth and the depth|id min for faces [0, T) of d_verts with ids id_base + f. */
+int tina_raster(tina_ctx *ctx, const float *d_verts, int nfaces, uint32_t id_base,
+                uint32_t flags, void *stream);
+
+/* TriangleRaster.render_color + Shader/Lighting/material (triangle.py:133-153,
+ * shader.py:112-131, lighting.py:84-98): shades the pixels whose winning id lies
+ * in [mesh->id_base, mesh->id_base + mesh->nfaces) into d_img ([H][W][3] f32).
+ * With TINA_BG also writes bg to empty pixels; with TINA_TONEMAP applies ACES. */
+int tina_shade(tina_ctx *ctx, const tina_mesh *mesh, const tina_material *mat,
+               const tina_lights *lights, const tina_ibl *ibl, float *d_img,
+               const float bg[3], uint32_t flags, void *stream);
+
+/* Scene.image.fill(bgcolor) (tina/scene/raster.py:176), ACES-mapped with TINA_TONEMAP. */
+int tina_fill(tina_ctx *ctx, float *d_img, const float bg[3], uint32_t flags, void *stream);
+
+/* ToneMapping.apply (tina/postp/tonemap.py:9-12): in-place ACES over n RGB pixels. */
+int tina_tonemap_aces(float *d_img, int npixels, void *stream);
+
+/* Engine.depth and TriangleRaster.occup / ProbeShader.elmid readback
+ * (engine.py:9, triangle.py:16, tina/probe.py:17-20): unpacks the keys into
+ * d_id ([H][W] i32, winning global id or -1) and d_depth ([H][W] i32, 2**30 when
+ * empty).  Either output may be NULL. */
+int tina_resolve(tina_ctx *ctx, int32_t *d_id, int32_t *d_depth, void *stream);
+
+/* Device pointer of the packed u64 key buffer ([H][W]) for zero-copy consumers. */
+int tina_keys(tina_ctx *ctx, uint64_t **d_keys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TINA_HIP_H */

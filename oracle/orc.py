@@ -1,0 +1,148 @@
+"""ctypes wrapper of the CPU oracle (liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product package never
+imports it.  The restatement it wraps is described in tina_oracle.h; what pins
+each function is listed in oracle/README.md.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, 'liborc.so')
+
+_lib = None
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.check_call(['make', '-s', '-C', HERE, 'liborc.so'] + (['-B'] if force else []))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(0 if a is None else a.ctypes.data)
+
+
+def camera(W2V, V2W, bias=(0.5, 0.5)):
+    from taichi_three_amd import _abi
+    cam = _abi.Camera()
+    cam.W2V[:] = [float(x) for x in np.asarray(W2V, dtype=np.float32).reshape(-1)]
+    cam.V2W[:] = [float(x) for x in np.asarray(V2W, dtype=np.float32).reshape(-1)]
+    cam.bias[:] = [float(bias[0]), float(bias[1])]
+    return cam
+
+
+def raster(W, H, cam, verts, id_base=0, flags=3, depth=None, occup=None):
+    """Serial coverage (orc_raster).  Returns (depth, occup) [H, W] i32."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    if depth is None:
+        depth = np.empty((H, W), dtype=np.int32)
+        occup = np.empty((H, W), dtype=np.int32)
+        lib().orc_clear(W, H, _p(depth), _p(occup))
+    lib().orc_raster(W, H, ctypes.byref(cam), _p(verts), len(verts), ctypes.c_uint32(id_base),
+                     ctypes.c_uint32(flags), _p(depth), _p(occup))
+    return depth, occup
+
+
+def raster_omp(W, H, cam, verts, id_base=0, flags=3, keys=None, nthreads=1):
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    if keys is None:
+        keys = np.empty((H, W), dtype=np.uint64)
+        lib().orc_clear_keys(W, H, _p(keys), nthreads)
+    lib().orc_raster_omp(W, H, ctypes.byref(cam), _p(verts), len(verts), ctypes.c_uint32(id_base),
+                         ctypes.c_uint32(flags), _p(keys), nthreads)
+    return keys
+
+
+def unpack(keys):
+    H, W = keys.shape
+    depth = np.empty((H, W), dtype=np.int32)
+    occup = np.empty((H, W), dtype=np.int32)
+    lib().orc_unpack_keys(W, H, _p(keys), _p(depth), _p(occup))
+    return depth, occup
+
+
+class HostMesh:
+    """tina_mesh over host arrays (kept alive by this object)."""
+
+    def __init__(self, verts, norms=None, coors=None, colors=None, id_base=0):
+        from taichi_three_amd import _abi
+        self.arrays = [None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+                       for a in (verts, norms, coors, colors)]
+        m = _abi.Mesh()
+        m.verts, m.norms, m.coors, m.colors = [0 if a is None else a.ctypes.data for a in self.arrays]
+        m.nfaces = len(verts)
+        m.id_base = id_base
+        self.struct = m
+
+
+def shade(W, H, cam, mesh, material, lights, occup, flags, bg=(0, 0, 0), ibl=None, f64=False, nthreads=8):
+    """Deferred shading of the pixels of `mesh` (orc_shade_f32 / f64).  Returns
+    the image [H, W, 3] (pixels outside the mesh's id range are 0 unless BG)."""
+    occup = np.ascontiguousarray(occup, dtype=np.int32)
+    if f64:
+        img = np.zeros((H, W, 3), dtype=np.float64)
+        bga = (ctypes.c_double * 3)(*[float(x) for x in np.broadcast_to(bg, 3)])
+        fn = lib().orc_shade_f64
+    else:
+        img = np.zeros((H, W, 3), dtype=np.float32)
+        bga = (ctypes.c_float * 3)(*[float(x) for x in np.broadcast_to(bg, 3)])
+        fn = lib().orc_shade_f32
+    fn(W, H, ctypes.byref(cam), ctypes.byref(mesh.struct), ctypes.byref(material), ctypes.byref(lights),
+       ctypes.byref(ibl) if ibl is not None else None, _p(occup), _p(img), bga, ctypes.c_uint32(flags), nthreads)
+    return img
+
+
+def shade_point(material, lights, pos, nrm, viewdir, flags=0, ibl=None, slots=None, f64=True):
+    """Shade one point (no tone mapping): Lighting.shade_color(material, pos, nrm, viewdir)."""
+    if f64:
+        T, fn, ct = np.float64, lib().orc_shade_point_f64, ctypes.c_double
+    else:
+        T, fn, ct = np.float32, lib().orc_shade_point_f32, ctypes.c_float
+    a = [np.ascontiguousarray(x, dtype=T) for x in (pos, nrm, viewdir)]
+    s = None if slots is None else np.ascontiguousarray(slots, dtype=T)
+    out = np.zeros(3, dtype=T)
+    fn(ctypes.byref(material), ctypes.byref(lights), ctypes.byref(ibl) if ibl is not None else None,
+       ctypes.c_uint32(flags), _p(a[0]), _p(a[1]), _p(a[2]), _p(s), _p(out))
+    return out
+
+
+def aces(img, f64=False):
+    img = np.array(img, dtype=np.float64 if f64 else np.float32, order='C')
+    n = img.size // 3
+    (lib().orc_aces_f64 if f64 else lib().orc_aces_f32)(_p(img), n)
+    return img
+
+
+def skybox_sample(img, d):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros(3)
+    lib().orc_skybox_sample_f64(_p(img), img.shape[0], img.shape[1], _p(np.asarray(d, dtype=np.float64)), _p(out))
+    return out
+
+
+def bake_lambert(env, ow, oh, nsamples, nthreads=8):
+    env = np.ascontiguousarray(env, dtype=np.float32)
+    out = np.zeros((ow, oh, 3), dtype=np.float32)
+    lib().orc_bake_lambert(_p(env), env.shape[0], env.shape[1], _p(out), ow, oh, nsamples, nthreads)
+    return out
+
+
+def bake_ggx(env, ow, oh, roughness, nsamples, nthreads=8):
+    env = np.ascontiguousarray(env, dtype=np.float32)
+    out = np.zeros((ow, oh, 3), dtype=np.float32)
+    lib().orc_bake_ggx(_p(env), env.shape[0], env.shape[1], _p(out), ow, oh, ctypes.c_float(roughness),
+                       nsamples, nthreads)
+    return out

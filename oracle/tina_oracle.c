@@ -1,0 +1,322 @@
+/*
+ * tina_oracle.c -- TEST INFRASTRUCTURE: the CPU checker of the HIP path.
+ * Coverage part (orc_raster*).  Shading lives in shade_impl.h, instantiated
+ * for float and double at the bottom of this file.
+ *
+ * Coverage restates tina/core/triangle.py:88-131 step by step (the comments give
+ * the line each step follows).  All arithmetic is binary32, evaluated left to
+ * right as the Python source groups it, with no FMA (-ffp-contract=off).
+ * Conversions float -> int saturate (NaN -> 0), the behaviour of the AMD
+ * conversion instruction and of Taichi's CUDA backend; Taichi's x86 backend
+ * would give INT_MIN out of range, which no workload here reaches.
+ */
+#include "tina_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#define MAXDEPTH 1073741824 /* Engine.maxdepth = 2**30, tina/core/engine.py:12 */
+
+static inline int32_t f2i_sat(float x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)x;
+}
+
+/* mapply(mat, pos, 1) (tina/common.py:169-177): r_i = M[i][3]*1 then += M[i][j]*p_j
+ * for j = 0,1,2; w likewise from row 3. */
+static inline void mapply1(const float *M, const float *p, float r[3], float *w)
+{
+    for (int i = 0; i < 3; i++) {
+        float acc = M[4 * i + 3] * 1.0f;
+        acc = acc + M[4 * i + 0] * p[0];
+        acc = acc + M[4 * i + 1] * p[1];
+        acc = acc + M[4 * i + 2] * p[2];
+        r[i] = acc;
+    }
+    float ww = M[15] * 1.0f;
+    ww = ww + M[12] * p[0];
+    ww = ww + M[13] * p[1];
+    ww = ww + M[14] * p[2];
+    *w = ww;
+}
+
+typedef struct {
+    float Av[3], Bv[3], Cv[3];
+    float b[2], c[2];
+    float bcn[2], can[2], wsc[3];
+    int32_t bot[2], top[2];
+} face_setup;
+
+/* Per-face part of render_occup (triangle.py:93-113).  Returns 0 when the face
+ * is culled or clipped. */
+static int setup_face(const tina_camera *cam, int W, int H, const float *v,
+                      uint32_t flags, face_setup *s)
+{
+    const float *M = cam->W2V;
+    float r[3][3], w[3];
+    for (int k = 0; k < 3; k++) mapply1(M, v + 3 * k, r[k], &w[k]);
+    float *V[3] = {s->Av, s->Bv, s->Cv};
+    for (int k = 0; k < 3; k++)                       /* mapply_pos: res / rew */
+        for (int i = 0; i < 3; i++) V[k][i] = r[k][i] / w[k];
+
+    /* facing = (Bv.xy - Av.xy).cross(Cv.xy - Av.xy)   (:95) */
+    float facing = (s->Bv[0] - s->Av[0]) * (s->Cv[1] - s->Av[1])
+                 - (s->Bv[1] - s->Av[1]) * (s->Cv[0] - s->Av[0]);
+    if ((flags & TINA_CULL) && facing <= 0.0f) return 0;          /* :96-98 */
+
+    if (flags & TINA_CLIP) {                                       /* :100-104 */
+        int outside = 0;
+        for (int k = 0; k < 3; k++) {
+            int inside = 1;
+            for (int i = 0; i < 3; i++)
+                inside &= (-1.0f <= V[k][i]) & (V[k][i] <= 1.0f);
+            outside += !inside;
+        }
+        if (outside == 3) return 0;
+    }
+
+    /* to_viewport: (p.xy * 0.5 + 0.5) * res   (engine.py:59-61) */
+    float a[2], b[2], c[2];
+    const float res[2] = {(float)W, (float)H};
+    for (int i = 0; i < 2; i++) {
+        a[i] = (s->Av[i] * 0.5f + 0.5f) * res[i];
+        b[i] = (s->Bv[i] * 0.5f + 0.5f) * res[i];
+        c[i] = (s->Cv[i] * 0.5f + 0.5f) * res[i];
+    }
+    /* bot, top = ifloor(min(a,b,c)), iceil(max(a,b,c)); clamp to [0, res-1] (:108-109) */
+    const int32_t lim[2] = {W - 1, H - 1};
+    for (int i = 0; i < 2; i++) {
+        float lo = fminf(a[i], fminf(b[i], c[i]));
+        float hi = fmaxf(a[i], fmaxf(b[i], c[i]));
+        int32_t bo = f2i_sat(floorf(lo));
+        int32_t to = f2i_sat(ceilf(hi));
+        s->bot[i] = bo > 0 ? bo : 0;
+        s->top[i] = to < lim[i] ? to : lim[i];
+    }
+    /* n = (b - a).cross(c - a); bcn = (b - c) / n; can = (c - a) / n   (:110-112) */
+    float n = (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
+    s->bcn[0] = (b[0] - c[0]) / n;
+    s->bcn[1] = (b[1] - c[1]) / n;
+    s->can[0] = (c[0] - a[0]) / n;
+    s->can[1] = (c[1] - a[1]) / n;
+    s->b[0] = b[0]; s->b[1] = b[1];
+    s->c[0] = c[0]; s->c[1] = c[1];
+    /* wscale = 1 / (w_A, w_B, w_C)   (:113) */
+    for (int k = 0; k < 3; k++) s->wsc[k] = 1.0f / w[k];
+    return 1;
+}
+
+/* Per-pixel weights (:115-120) and the coverage test (:120). */
+static inline int pixel_cover(const face_setup *s, int32_t px, int32_t py,
+                              const float bias[2], float wei[3])
+{
+    float posx = (float)px + bias[0];
+    float posy = (float)py + bias[1];
+    float w_bc = (posx - s->b[0]) * s->bcn[1] - (posy - s->b[1]) * s->bcn[0];
+    float w_ca = (posx - s->c[0]) * s->can[1] - (posy - s->c[1]) * s->can[0];
+    float t = (1.0f - w_bc) - w_ca;
+    float x = w_bc * s->wsc[0];
+    float y = w_ca * s->wsc[1];
+    float z = t * s->wsc[2];
+    float sum = (x + y) + z;
+    wei[0] = x / sum;
+    wei[1] = y / sum;
+    wei[2] = z / sum;
+    return (wei[0] >= 0.0f) & (wei[1] >= 0.0f) & (wei[2] >= 0.0f);
+}
+
+/* depth = int((wei.x*Av.z + wei.y*Bv.z + wei.z*Cv.z) * 2**30)   (:121-122) */
+static inline int32_t pixel_depth(const face_setup *s, const float wei[3])
+{
+    float d = ((wei[0] * s->Av[2]) + (wei[1] * s->Bv[2])) + (wei[2] * s->Cv[2]);
+    return f2i_sat(d * 1073741824.0f);
+}
+
+void orc_clear(int W, int H, int32_t *depth, int32_t *occup)
+{
+    for (long i = 0; i < (long)W * H; i++) {
+        depth[i] = MAXDEPTH;       /* engine.py:67-70 */
+        occup[i] = -1;             /* triangle.py:90-91 */
+    }
+}
+
+void orc_raster(int W, int H, const tina_camera *cam, const float *verts, int T,
+                uint32_t id_base, uint32_t flags, int32_t *depth, int32_t *occup)
+{
+    for (int f = 0; f < T; f++) {
+        face_setup s;
+        if (!setup_face(cam, W, H, verts + 9L * f, flags, &s)) continue;
+        for (int32_t px = s.bot[0]; px <= s.top[0]; px++)
+            for (int32_t py = s.bot[1]; py <= s.top[1]; py++) {
+                float wei[3];
+                if (!pixel_cover(&s, px, py, cam->bias, wei)) continue;
+                int32_t d = pixel_depth(&s, wei);
+                long P = (long)py * W + px;
+                if (d < depth[P]) {          /* serial form of atomic_min + occup (:123-125) */
+                    depth[P] = d;
+                    occup[P] = (int32_t)(id_base + (uint32_t)f);
+                }
+            }
+    }
+}
+
+static inline uint64_t pack_key(int32_t depth, uint32_t id)
+{
+    return ((uint64_t)((uint32_t)depth ^ 0x80000000u) << 32) | (uint64_t)id;
+}
+
+void orc_clear_keys(int W, int H, uint64_t *keys, int nthreads)
+{
+    uint64_t k = pack_key(MAXDEPTH, 0);
+    long n = (long)W * H;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long i = 0; i < n; i++) keys[i] = k;
+}
+
+void orc_raster_omp(int W, int H, const tina_camera *cam, const float *verts, int T,
+                    uint32_t id_base, uint32_t flags, uint64_t *keys, int nthreads)
+{
+    /* One task per face, each scanning its bbox serially: the reference's
+     * launch shape (triangle.py:92-125), with the race removed by a 64-bit
+     * compare-and-swap min on the packed (depth, id) key. */
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+    for (int f = 0; f < T; f++) {
+        face_setup s;
+        if (!setup_face(cam, W, H, verts + 9L * f, flags, &s)) continue;
+        uint32_t id = id_base + (uint32_t)f;
+        for (int32_t px = s.bot[0]; px <= s.top[0]; px++)
+            for (int32_t py = s.bot[1]; py <= s.top[1]; py++) {
+                float wei[3];
+                if (!pixel_cover(&s, px, py, cam->bias, wei)) continue;
+                uint64_t k = pack_key(pixel_depth(&s, wei), id);
+                uint64_t *dst = keys + (long)py * W + px;
+                uint64_t cur = __atomic_load_n(dst, __ATOMIC_RELAXED);
+                while (k < cur &&
+                       !__atomic_compare_exchange_n(dst, &cur, k, 1, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED)) {
+                }
+            }
+    }
+}
+
+void orc_unpack_keys(int W, int H, const uint64_t *keys, int32_t *depth, int32_t *occup)
+{
+    for (long i = 0; i < (long)W * H; i++) {
+        int32_t d = (int32_t)((uint32_t)(keys[i] >> 32) ^ 0x80000000u);
+        if (depth) depth[i] = d;
+        if (occup) occup[i] = d >= MAXDEPTH ? -1 : (int32_t)(uint32_t)keys[i];
+    }
+}
+
+/* ---------------------------------------------------------------- shading */
+#define REAL float
+#define RSUF(name) name##_f32
+#define RC(x) ((float)(x))
+#include "shade_impl.h"
+#undef REAL
+#undef RSUF
+#undef RC
+
+#define REAL double
+#define RSUF(name) name##_f64
+#define RC(x) ((double)(x))
+#include "shade_impl.h"
+#undef REAL
+#undef RSUF
+#undef RC
+
+/* ---------------------------------------------------------------- IBL bakes */
+
+
+void orc_skybox_sample_f64(const float *img, int w, int h, const double dir[3], double out[3])
+{
+    vec_f64 r = skybox_sample_f64(img, w, h, v3_f64(dir[0], dir[1], dir[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* Van der Corput radical inverse in base 2: the second Hammersley coordinate. */
+static double radical_inverse2(uint32_t i)
+{
+    i = (i << 16) | (i >> 16);
+    i = ((i & 0x55555555u) << 1) | ((i & 0xAAAAAAAAu) >> 1);
+    i = ((i & 0x33333333u) << 2) | ((i & 0xCCCCCCCCu) >> 2);
+    i = ((i & 0x0F0F0F0Fu) << 4) | ((i & 0xF0F0F0F0u) >> 4);
+    i = ((i & 0x00FF00FFu) << 8) | ((i & 0xFF00FF00u) >> 8);
+    return (double)i * 2.3283064365386963e-10;
+}
+
+/* Skybox.unmapcoor (tina/skybox.py:131-136) + unspheremap (:31-35) + spherical
+ * (tina/advans.py:104-108). */
+static vec_f64 unmapcoor_f64(int ix, int iy, int w, int h)
+{
+    const double tau = 6.283185307179586;
+    double v = (double)ix / (double)(w - 1), u = (double)iy / (double)(h - 1);
+    double hh = u * 2.0 - 1.0;
+    double r = sqrt(fmax(0.0, 1.0 - hh * hh));
+    double dx = r * cos(v * tau), dy = r * sin(v * tau), dz = hh;
+    return v3_f64(dx, dz, -dy);
+}
+
+/* tangentspace(nrm) @ spherical(u, v) (tina/advans.py:96-108) */
+static vec_f64 tangent_dir_f64(vec_f64 nrm, double u, double v)
+{
+    const double tau = 6.283185307179586;
+    vec_f64 up = v3_f64(233.0, 666.0, 512.0);
+    vec_f64 bitan = vnormalized_f64(vcross_f64(nrm, up));
+    vec_f64 tan = vcross_f64(bitan, nrm);
+    double r = sqrt(fmax(0.0, 1.0 - u * u));
+    double sx = r * cos(v * tau), sy = r * sin(v * tau), sz = u;
+    return vadd_f64(vadd_f64(vsmul_f64(sx, tan), vsmul_f64(sy, bitan)), vsmul_f64(sz, nrm));
+}
+
+/* Lambert.cook_for_ibl bake (material.py:414-426) with Hammersley points in place
+ * of ti.random: res += env(odir) * u; dem += u; out = res / dem. */
+void orc_bake_lambert(const float *env, int ew, int eh, float *out, int ow, int oh, int nsamples, int nthreads)
+{
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4) collapse(2)
+    for (int ix = 0; ix < ow; ix++)
+        for (int iy = 0; iy < oh; iy++) {
+            vec_f64 dir = unmapcoor_f64(ix, iy, ow, oh);
+            vec_f64 res = vs_f64(0.0);
+            double dem = 0.0;
+            for (int s = 0; s < nsamples; s++) {
+                double u = ((double)s + 0.5) / (double)nsamples, v = radical_inverse2((uint32_t)s);
+                vec_f64 odir = tangent_dir_f64(dir, u, v);
+                vec_f64 wei = skybox_sample_f64(env, ew, eh, odir);
+                res = vadd_f64(res, vsmul_f64(u, wei));
+                dem += u;
+            }
+            float *o = out + ((long)ix * oh + iy) * 3;
+            o[0] = (float)(res.x / dem); o[1] = (float)(res.y / dem); o[2] = (float)(res.z / dem);
+        }
+}
+
+/* CookTorrance.cook_for_ibl bake (material.py:283-299): GGX-distributed cosines
+ * u = sqrt((1 - u) / (1 - u * (1 - alpha2))), weights u. */
+void orc_bake_ggx(const float *env, int ew, int eh, float *out, int ow, int oh,
+                  float roughness, int nsamples, int nthreads)
+{
+    double alpha2 = fmax(0.0, (double)roughness * (double)roughness);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4) collapse(2)
+    for (int ix = 0; ix < ow; ix++)
+        for (int iy = 0; iy < oh; iy++) {
+            vec_f64 dir = unmapcoor_f64(ix, iy, ow, oh);
+            vec_f64 res = vs_f64(0.0);
+            double dem = 0.0;
+            for (int s = 0; s < nsamples; s++) {
+                double u0 = ((double)s + 0.5) / (double)nsamples, v = radical_inverse2((uint32_t)s);
+                double u = sqrt((1.0 - u0) / (1.0 - u0 * (1.0 - alpha2)));
+                vec_f64 odir = tangent_dir_f64(dir, u, v);
+                vec_f64 wei = skybox_sample_f64(env, ew, eh, odir);
+                res = vadd_f64(res, vsmul_f64(u, wei));
+                dem += u;
+            }
+            float *o = out + ((long)ix * oh + iy) * 3;
+            o[0] = (float)(res.x / dem); o[1] = (float)(res.y / dem); o[2] = (float)(res.z / dem);
+        }
+}

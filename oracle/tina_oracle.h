@@ -1,0 +1,86 @@
+/*
+ * tina_oracle.h -- CPU restatement of Tina's rasterise-and-shade path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+ * product (taichi_three_amd + libtina_hip.so) never links or calls it.
+ *
+ * It restates, in IEEE binary32 with no FMA contraction (built with
+ * -ffp-contract=off) and serial per-face order, the Taichi kernels of the
+ * reference:
+ *   orc_raster        tina/core/triangle.py:88-131 (render_occup) with the serial
+ *                     update rule "depth < depth_buf -> take it", faces ascending
+ *   orc_shade_f32/f64 tina/core/triangle.py:133-153 (render_color, interpolate :31-49),
+ *                     tina/core/shader.py:81-93,112-131, tina/core/lighting.py:84-98,
+ *                     tina/matr/material.py (Lambert, Phong, CookTorrance, Mix,
+ *                     Scale, Add, FresnelFactor), tina/matr/nodes.py:42-111,
+ *                     tina/postp/tonemap.py + tina/advans.py:31-34 (ACES)
+ *   orc_ibl_*         tina/core/lighting.py:4-21, material.py:251-320,407-442,
+ *                     tina/skybox.py:23-145, tina/common.py:140-149 (bilerp)
+ *   orc_raster_omp    the same coverage rule, parallel over faces with a 64-bit
+ *                     compare-and-swap min (the CPU baseline; same result)
+ *
+ * Pinning status of each function is listed in oracle/README.md.
+ */
+#ifndef TINA_ORACLE_H
+#define TINA_ORACLE_H
+#include <stdint.h>
+#include "../include/tina_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Serial coverage.  depth/occup are [H][W] i32, in/out (clear with orc_clear). */
+void orc_clear(int W, int H, int32_t *depth, int32_t *occup);
+void orc_raster(int W, int H, const tina_camera *cam, const float *verts, int T,
+                uint32_t id_base, uint32_t flags, int32_t *depth, int32_t *occup);
+
+/* Parallel coverage into packed keys ([H][W] u64), nthreads OpenMP threads. */
+void orc_clear_keys(int W, int H, uint64_t *keys, int nthreads);
+void orc_raster_omp(int W, int H, const tina_camera *cam, const float *verts, int T,
+                    uint32_t id_base, uint32_t flags, uint64_t *keys, int nthreads);
+void orc_unpack_keys(int W, int H, const uint64_t *keys, int32_t *depth, int32_t *occup);
+
+/* Host-side mesh description (pointers are host addresses cast to uint64). */
+/* Shading of the pixels whose id lies in [mesh->id_base, +nfaces).  img [H][W][3].
+ * f32: op-exact binary32; f64: the same formulas in binary64 (written as f64 too). */
+void orc_shade_f32(int W, int H, const tina_camera *cam, const tina_mesh *mesh,
+                   const tina_material *mat, const tina_lights *lights,
+                   const tina_ibl *ibl, const int32_t *occup, float *img,
+                   const float bg[3], uint32_t flags, int nthreads);
+void orc_shade_f64(int W, int H, const tina_camera *cam, const tina_mesh *mesh,
+                   const tina_material *mat, const tina_lights *lights,
+                   const tina_ibl *ibl, const int32_t *occup, double *img,
+                   const double bg[3], uint32_t flags, int nthreads);
+
+/* ACES (tina/advans.py:31-34), in place over n RGB pixels. */
+void orc_aces_f32(float *img, int npixels);
+void orc_aces_f64(double *img, int npixels);
+
+/* Single-sample entry points used by the closed-form pinning tests and by the
+ * neural-surrogate training-set generator: shade one (n, view, pos) with the
+ * material and lights, no tone mapping.  params_override (may be NULL) gives
+ * the per-sample value of each parameter slot as float3. */
+void orc_shade_point_f64(const tina_material *mat, const tina_lights *lights,
+                         const tina_ibl *ibl, uint32_t flags, const double pos[3],
+                         const double nrm[3], const double viewdir[3],
+                         const double *slot_values /* [nparams][3] or NULL */,
+                         double out[3]);
+void orc_shade_point_f32(const tina_material *mat, const tina_lights *lights,
+                         const tina_ibl *ibl, uint32_t flags, const float pos[3],
+                         const float nrm[3], const float viewdir[3],
+                         const float *slot_values, float out[3]);
+
+/* IBL helpers: Skybox.sample (tina/skybox.py:138-145) and the deterministic
+ * Hammersley-quadrature bakes that stand in for the ti.random bakes
+ * (material.py:283-299, 414-426). env/out are [w][h][3] (Skybox.img layout). */
+void orc_skybox_sample_f64(const float *img, int w, int h, const double dir[3], double out[3]);
+void orc_bake_lambert(const float *env, int ew, int eh, float *out, int ow, int oh, int nsamples, int nthreads);
+void orc_bake_ggx(const float *env, int ew, int eh, float *out, int ow, int oh,
+                  float roughness, int nsamples, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,128 @@
+// tina_dev.h -- device-side building blocks shared by the raster and shade kernels.
+//
+// Every float expression keeps the grouping of the Tina source it follows and is
+// compiled with -ffp-contract=off, so each multiply and add rounds separately:
+// the coverage test and the packed depth are then bit-identical to the binary32
+// evaluation of tina/core/triangle.py:88-131.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/tina_hip.h"
+
+#define TINA_MAXDEPTH 1073741824  // Engine.maxdepth = 2**30 (tina/core/engine.py:12)
+
+namespace tina {
+
+// float -> int32 with saturation and NaN -> 0 (the v_cvt_i32_f32 rule, spelled out
+// so the compiler never treats an out-of-range value as undefined).
+__device__ __forceinline__ int32_t f2i_sat(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x <= -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)x;
+}
+
+__device__ __forceinline__ uint64_t pack_key(int32_t depth, uint32_t id) {
+  return ((uint64_t)((uint32_t)depth ^ 0x80000000u) << 32) | (uint64_t)id;
+}
+__host__ __device__ __forceinline__ uint64_t empty_key() {
+  return ((uint64_t)((uint32_t)TINA_MAXDEPTH ^ 0x80000000u) << 32);
+}
+__device__ __forceinline__ int32_t key_depth(uint64_t k) {
+  return (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+}
+
+// Camera passed by value as a kernel argument (lives in the kernarg segment).
+struct Cam {
+  float W2V[16];
+  float V2W[16];
+  float bias[2];
+  int W, H;
+};
+
+// mapply(mat, pos, 1) (tina/common.py:169-177)
+__device__ __forceinline__ void mapply1(const float* __restrict__ M, float p0, float p1, float p2,
+                                        float& r0, float& r1, float& r2, float& w) {
+  float a = M[3] * 1.0f;  a = a + M[0] * p0;  a = a + M[1] * p1;  a = a + M[2] * p2;  r0 = a;
+  float b = M[7] * 1.0f;  b = b + M[4] * p0;  b = b + M[5] * p1;  b = b + M[6] * p2;  r1 = b;
+  float c = M[11] * 1.0f; c = c + M[8] * p0;  c = c + M[9] * p1;  c = c + M[10] * p2; r2 = c;
+  float d = M[15] * 1.0f; d = d + M[12] * p0; d = d + M[13] * p1; d = d + M[14] * p2; w = d;
+}
+
+// The per-face state render_occup keeps (triangle.py:127-131) plus what the
+// coverage loop needs: viewport b, c; bcn, can; wscale; view-space z of A, B, C.
+struct FaceSetup {
+  float bx, by, cx, cy;
+  float bcnx, bcny, canx, cany;
+  float ws0, ws1, ws2;
+  float Az, Bz, Cz;
+  int32_t botx, boty, topx, topy;
+};
+
+// Per-face part of render_occup (triangle.py:93-113).  Returns false when the
+// face is culled or clipped.  v points at 9 floats (A, B, C).
+__device__ __forceinline__ bool setup_face(const Cam& cam, const float* v, uint32_t flags,
+                                           FaceSetup& s) {
+  float Ar0, Ar1, Ar2, Aw, Br0, Br1, Br2, Bw, Cr0, Cr1, Cr2, Cw;
+  mapply1(cam.W2V, v[0], v[1], v[2], Ar0, Ar1, Ar2, Aw);
+  mapply1(cam.W2V, v[3], v[4], v[5], Br0, Br1, Br2, Bw);
+  mapply1(cam.W2V, v[6], v[7], v[8], Cr0, Cr1, Cr2, Cw);
+  const float Avx = Ar0 / Aw, Avy = Ar1 / Aw, Avz = Ar2 / Aw;
+  const float Bvx = Br0 / Bw, Bvy = Br1 / Bw, Bvz = Br2 / Bw;
+  const float Cvx = Cr0 / Cw, Cvy = Cr1 / Cw, Cvz = Cr2 / Cw;
+
+  const float facing = (Bvx - Avx) * (Cvy - Avy) - (Bvy - Avy) * (Cvx - Avx);  // :95
+  if ((flags & TINA_CULL) && facing <= 0.0f) return false;                     // :96-98
+  if (flags & TINA_CLIP) {                                                      // :100-104
+    const bool ai = (-1.0f <= Avx) & (Avx <= 1.0f) & (-1.0f <= Avy) & (Avy <= 1.0f) &
+                    (-1.0f <= Avz) & (Avz <= 1.0f);
+    const bool bi = (-1.0f <= Bvx) & (Bvx <= 1.0f) & (-1.0f <= Bvy) & (Bvy <= 1.0f) &
+                    (-1.0f <= Bvz) & (Bvz <= 1.0f);
+    const bool ci = (-1.0f <= Cvx) & (Cvx <= 1.0f) & (-1.0f <= Cvy) & (Cvy <= 1.0f) &
+                    (-1.0f <= Cvz) & (Cvz <= 1.0f);
+    if (!ai && !bi && !ci) return false;
+  }
+  const float resx = (float)cam.W, resy = (float)cam.H;
+  const float ax = (Avx * 0.5f + 0.5f) * resx, ay = (Avy * 0.5f + 0.5f) * resy;  // to_viewport
+  const float bx = (Bvx * 0.5f + 0.5f) * resx, by = (Bvy * 0.5f + 0.5f) * resy;
+  const float cx = (Cvx * 0.5f + 0.5f) * resx, cy = (Cvy * 0.5f + 0.5f) * resy;
+
+  const int32_t bo_x = f2i_sat(floorf(fminf(ax, fminf(bx, cx))));               // :108-109
+  const int32_t bo_y = f2i_sat(floorf(fminf(ay, fminf(by, cy))));
+  const int32_t to_x = f2i_sat(ceilf(fmaxf(ax, fmaxf(bx, cx))));
+  const int32_t to_y = f2i_sat(ceilf(fmaxf(ay, fmaxf(by, cy))));
+  s.botx = bo_x > 0 ? bo_x : 0;
+  s.boty = bo_y > 0 ? bo_y : 0;
+  s.topx = to_x < cam.W - 1 ? to_x : cam.W - 1;
+  s.topy = to_y < cam.H - 1 ? to_y : cam.H - 1;
+
+  const float n = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);               // :110-112
+  s.bcnx = (bx - cx) / n;
+  s.bcny = (by - cy) / n;
+  s.canx = (cx - ax) / n;
+  s.cany = (cy - ay) / n;
+  s.bx = bx; s.by = by; s.cx = cx; s.cy = cy;
+  s.ws0 = 1.0f / Aw; s.ws1 = 1.0f / Bw; s.ws2 = 1.0f / Cw;                       // :113
+  s.Az = Avz; s.Bz = Bvz; s.Cz = Cvz;
+  return true;
+}
+
+// Weights (:115-119) for pixel (px, py); returns the coverage test (:120).
+__device__ __forceinline__ bool pixel_weights(const FaceSetup& s, float posx, float posy,
+                                              float& w0, float& w1, float& w2) {
+  const float w_bc = (posx - s.bx) * s.bcny - (posy - s.by) * s.bcnx;
+  const float w_ca = (posx - s.cx) * s.cany - (posy - s.cy) * s.canx;
+  const float t = (1.0f - w_bc) - w_ca;
+  const float x = w_bc * s.ws0, y = w_ca * s.ws1, z = t * s.ws2;
+  const float sum = (x + y) + z;
+  w0 = x / sum; w1 = y / sum; w2 = z / sum;
+  return (w0 >= 0.0f) & (w1 >= 0.0f) & (w2 >= 0.0f);
+}
+
+// depth = int((wei.x*Av.z + wei.y*Bv.z + wei.z*Cv.z) * 2**30)   (:121-122)
+__device__ __forceinline__ int32_t pixel_depth(const FaceSetup& s, float w0, float w1, float w2) {
+  const float d = ((w0 * s.Az) + (w1 * s.Bz)) + (w2 * s.Cz);
+  return f2i_sat(d * 1073741824.0f);
+}
+
+}  // namespace tina

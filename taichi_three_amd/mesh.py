@@ -1,0 +1,320 @@
+"""Mesh nodes: the input side of TriangleRaster.set_object (tina/mesh/*.py).
+
+The reference's node protocol is a set of ti.func getters that set_object
+inlines and copies into the raster's own [maxfaces, 3] fields every frame
+(tina/core/triangle.py:71-86).  Here a mesh keeps its face data resident in HBM
+as [T, 3, 3] f32 tensors (the same AoS layout) and `face_buffers()` hands the
+raster those tensors directly, so no per-frame copy happens.
+
+Extensions beyond the reference (needed by the benchmark workloads):
+  set_face_colors(c[T,3,3])   per-vertex albedo, bound to Input('color')
+  set_face_attr(name, a[T] or a[T,3])   per-face arrays read by tina.FaceAttr(name)
+"""
+import numpy as np
+
+from .engine import MAX
+
+__all__ = ['SimpleMesh', 'PrimitiveMesh', 'MeshFlatNormal', 'MeshSmoothNormal', 'ConnectiveMesh',
+           'MeshTransform', 'MeshEditBase']
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _as_device(arr, shape_tail, device=None):
+    torch = _torch()
+    dev = device or torch.device('cuda')
+    if isinstance(arr, torch.Tensor):
+        t = arr.to(device=dev, dtype=torch.float32)
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(np.asarray(arr, dtype=np.float32)), device=dev)
+    if tuple(t.shape[1:]) != tuple(shape_tail):
+        raise ValueError(f'expected an array of shape [n, {", ".join(map(str, shape_tail))}], got {tuple(t.shape)}')
+    return t.contiguous()
+
+
+class SimpleMesh:
+    """Face-soup mesh (tina/mesh/simple.py:4-115)."""
+
+    def __init__(self, maxfaces=MAX, npolygon=3):
+        if npolygon != 3:
+            raise NotImplementedError('only triangle meshes (npolygon=3) are rasterised here')
+        self.maxfaces = int(maxfaces)
+        self.npolygon = npolygon
+        self.nfaces = 0
+        self.verts = None
+        self.norms = None
+        self.coors = None
+        self.colors = None
+        self.mtlids = None
+        self.attrs = {}
+        self.version = 0
+
+    def get_npolygon(self):
+        return self.npolygon
+
+    def get_nfaces(self):
+        return min(self.nfaces, self.maxfaces)
+
+    def pre_compute(self):
+        pass
+
+    def _n(self, arr):
+        return min(int(arr.shape[0]), self.maxfaces)
+
+    def set_face_verts(self, verts):
+        """verts: [nfaces, 3, 3]; the face count is verts.shape[0] (simple.py:53-67)."""
+        n = self._n(verts)
+        self.verts = _as_device(verts[:n], (3, 3))
+        self.nfaces = n
+        self.version += 1
+
+    def set_face_norms(self, norms):
+        self.norms = _as_device(norms[:self.nfaces], (3, 3))
+        self.version += 1
+
+    def set_face_coors(self, coors):
+        self.coors = _as_device(coors[:self.nfaces], (3, 2))
+        self.version += 1
+
+    def set_face_mtlids(self, mtlids):
+        self.mtlids = np.asarray(mtlids[:self.nfaces], dtype=np.int32)
+
+    def set_material_id(self, mtlid):
+        self.mtlids = np.full(self.nfaces, mtlid, dtype=np.int32)
+
+    def set_face_colors(self, colors):
+        """Per-vertex colours [nfaces, 3, 3] bound to Input('color') (extension)."""
+        self.colors = _as_device(colors[:self.nfaces], (3, 3))
+        self.version += 1
+
+    def set_face_attr(self, name, values):
+        """Per-face scalar [nfaces] or vec3 [nfaces, 3] array read by tina.FaceAttr(name)."""
+        torch = _torch()
+        v = values if isinstance(values, torch.Tensor) else torch.as_tensor(np.asarray(values, dtype=np.float32))
+        v = v.to(device='cuda', dtype=torch.float32)[:self.nfaces].contiguous()
+        if v.dim() == 2 and v.shape[1] not in (1, 3):
+            raise ValueError('face attributes must be [n] or [n, 3]')
+        self.attrs[name] = v
+        self.version += 1
+
+    def face_buffers(self):
+        return dict(verts=self.verts, norms=self.norms, coors=self.coors, colors=self.colors,
+                    attrs=self.attrs, nfaces=self.get_nfaces() if self.verts is not None else 0)
+
+
+class PrimitiveMesh(SimpleMesh):
+    """Procedural meshes (tina/mesh/prim.py:6-111); faces[f][k] = (vert, norm, coor)."""
+
+    def __init__(self, faces):
+        faces = np.asarray(faces, dtype=np.float32)
+        super().__init__(maxfaces=len(faces), npolygon=faces.shape[1])
+        self.set_face_verts(faces[:, :, 0])
+        self.set_face_norms(faces[:, :, 1])
+        self.set_face_coors(faces[:, :, 2, :2])
+
+    @staticmethod
+    def sphere_faces(lons=32, lats=24, rad=1):
+        """UV sphere face list (prim.py:21-47), as float64 (F, 3, 3, 3)."""
+        def at(lat, lon):
+            lat /= lats
+            lon /= lons
+            coor = lon, lat, 0
+            lat = (lat - 0.5) * np.pi
+            lon = (lon * 2 - 1) * np.pi
+            x = np.cos(lat) * np.cos(lon)
+            y = np.cos(lat) * np.sin(lon)
+            z = np.sin(lat)
+            return (x * rad, y * rad, z * rad), (x, y, z), coor
+
+        faces = []
+        for lat in range(lats):
+            for lon in range(lons):
+                v1 = at(lat, lon)
+                v2 = at(lat, lon + 1)
+                v3 = at(lat + 1, lon + 1)
+                v4 = at(lat + 1, lon)
+                if lat != 0:
+                    faces.append([v1, v2, v3])
+                if lat != lats - 1:
+                    faces.append([v3, v4, v1])
+        return np.array(faces, dtype=np.float64)
+
+    @classmethod
+    def sphere(cls, lons=32, lats=24, rad=1):
+        return cls(cls.sphere_faces(lons, lats, rad))
+
+    @classmethod
+    def cylinder(cls, lons=32, lats=4, rad=1, hei=2):
+        """Capped cylinder (prim.py:50-94)."""
+        def at(lat, lon, rad=rad):
+            lat /= lats
+            lon /= lons
+            coor = lon, lat, 0
+            lat = lat - 0.5
+            lon = (lon * 2 - 1) * np.pi
+            x = np.cos(lon)
+            y = np.sin(lon)
+            return (x * rad, y * rad, lat * hei), (x, y, 0), coor
+
+        faces = []
+        for lat in range(lats):
+            for lon in range(lons):
+                v1, v2, v3, v4 = at(lat, lon), at(lat, lon + 1), at(lat + 1, lon + 1), at(lat + 1, lon)
+                faces.append([v1, v2, v3])
+                faces.append([v3, v4, v1])
+        for lon in range(lons):
+            norm, coor = (0, 0, -1), (0, 0, -1)
+            faces.append([(at(0, lon + 1)[0], norm, coor), (at(0, lon)[0], norm, coor),
+                          ((0, 0, -hei / 2), norm, coor)])
+        for lon in range(lons):
+            norm, coor = (0, 0, 1), (0, 0, 1)
+            faces.append([((0, 0, hei / 2), norm, coor), (at(lats, lon)[0], norm, coor),
+                          (at(lats, lon + 1)[0], norm, coor)])
+        return cls(np.array(faces, dtype=np.float64))
+
+
+class MeshEditBase:
+    """Forwards everything to the wrapped mesh (tina/mesh/base.py)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+
+    def __getattr__(self, attr):
+        return getattr(self.mesh, attr)
+
+    def face_buffers(self):
+        return self.mesh.face_buffers()
+
+
+class MeshFlatNormal(MeshEditBase):
+    """Per-face normals normalized((b - a) x (c - a)) on every corner (tina/mesh/norm.py:5-14)."""
+
+    def face_buffers(self):
+        fb = dict(self.mesh.face_buffers())
+        v = fb['verts']
+        if v is None:
+            return fb
+        torch = _torch()
+        a, b, c = v[:, 0], v[:, 1], v[:, 2]
+        e1, e2 = b - a, c - a
+        n = torch.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1],
+                         e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                         e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], dim=1)
+        n = n * (1.0 / torch.sqrt((n * n).sum(dim=1, keepdim=True)))
+        fb['norms'] = n[:, None, :].expand(-1, 3, -1).contiguous()
+        return fb
+
+
+class ConnectiveMesh:
+    """Indexed mesh: vertex arrays plus faces[nf, 3] indices (tina/mesh/conn.py:4-95,
+    tina/mesh/model.py:4-73).  Faces are expanded into a device soup for the raster."""
+
+    def __init__(self, verts=None, faces=None, norms=None, coors=None, colors=None):
+        self.npolygon = 3
+        self.verts_v = self.faces = self.norms_v = self.coors_v = self.colors_v = None
+        self.attrs = {}
+        self.version = 0
+        if verts is not None:
+            self.set_mesh(verts, faces, norms, coors, colors)
+
+    def get_npolygon(self):
+        return 3
+
+    def get_nfaces(self):
+        return 0 if self.faces is None else int(self.faces.shape[0])
+
+    def get_max_vert_nindex(self):
+        return 0 if self.verts_v is None else int(self.verts_v.shape[0])
+
+    def set_mesh(self, verts, faces, norms=None, coors=None, colors=None):
+        torch = _torch()
+        self.verts_v = _as_device(verts, (3,))
+        self.faces = torch.as_tensor(np.asarray(faces, dtype=np.int64), device='cuda')
+        self.norms_v = None if norms is None else _as_device(norms, (3,))
+        self.coors_v = None if coors is None else _as_device(coors, (2,))
+        self.colors_v = None if colors is None else _as_device(colors, (3,))
+        self.version += 1
+
+    def set_vertex_colors(self, colors):
+        self.colors_v = _as_device(colors, (3,))
+        self.version += 1
+
+    def set_face_attr(self, name, values):
+        SimpleMesh.set_face_attr(self, name, values)
+
+    @property
+    def nfaces(self):
+        return self.get_nfaces()
+
+    def face_buffers(self):
+        f = self.faces
+        g = (lambda a: None if a is None else a[f].contiguous())
+        return dict(verts=g(self.verts_v), norms=g(self.norms_v), coors=g(self.coors_v),
+                    colors=g(self.colors_v), attrs=self.attrs, nfaces=self.get_nfaces())
+
+
+class MeshSmoothNormal(MeshEditBase):
+    """Area-blind vertex normals: sum of normalized face normals, renormalised
+    (tina/mesh/norm.py:17-55).  Needs an indexed mesh (ConnectiveMesh)."""
+
+    def __init__(self, mesh, cached=True):
+        super().__init__(mesh)
+        if not hasattr(mesh, 'faces') or not hasattr(mesh, 'verts_v'):
+            raise NotImplementedError('MeshSmoothNormal needs an indexed mesh (tina.ConnectiveMesh)')
+        self.cached = cached
+        self._norm = None
+        self._version = None
+
+    def update_normal(self):
+        torch = _torch()
+        v, f = self.mesh.verts_v, self.mesh.faces
+        a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+        e1, e2 = b - a, c - a
+        n = torch.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1],
+                         e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                         e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], dim=1)
+        n = n * (1.0 / torch.sqrt((n * n).sum(dim=1, keepdim=True)))
+        acc = torch.zeros_like(v)
+        for k in range(3):
+            acc.index_add_(0, f[:, k], n)
+        self._norm = acc * (1.0 / torch.sqrt((acc * acc).sum(dim=1, keepdim=True)))
+        self._version = self.mesh.version
+
+    def face_buffers(self):
+        if self._norm is None or not self.cached or self._version != self.mesh.version:
+            self.update_normal()
+        fb = dict(self.mesh.face_buffers())
+        fb['norms'] = self._norm[self.mesh.faces].contiguous()
+        return fb
+
+
+class MeshTransform(MeshEditBase):
+    """Applies a 4x4 transform to positions and its inverse transpose to normals
+    (tina/mesh/trans.py:5-40)."""
+
+    def __init__(self, mesh, trans=None):
+        super().__init__(mesh)
+        self.trans = np.eye(4)
+        if trans is not None:
+            self.set_transform(trans)
+
+    def set_transform(self, trans):
+        self.trans = np.asarray(trans, dtype=np.float64)
+
+    def face_buffers(self):
+        torch = _torch()
+        fb = dict(self.mesh.face_buffers())
+        if fb['verts'] is None:
+            return fb
+        M = torch.as_tensor(self.trans.astype(np.float32), device=fb['verts'].device)
+        v = fb['verts']
+        r = v @ M[:3, :3].T + M[:3, 3]
+        w = v @ M[3, :3] + M[3, 3]
+        fb['verts'] = (r / w[..., None]).contiguous()
+        if fb['norms'] is not None:
+            N = torch.as_tensor(np.linalg.inv(self.trans)[:3, :3].T.astype(np.float32), device=v.device)
+            fb['norms'] = (fb['norms'] @ N.T).contiguous()
+        return fb

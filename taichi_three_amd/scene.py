@@ -1,0 +1,159 @@
+"""Scene (tina/scene/raster.py:4-258): the orchestrator scene.render() drops into.
+
+render() keeps the reference's order -- clear, then per object set_object /
+render_occup / render_color, then tone mapping -- but launches it the MI355X
+way: all coverage passes first into the shared depth|id buffer, then one
+shade launch per object over the pixels that object won, with the background
+fill and ACES fused into those launches.  The image equals the reference's:
+every pixel ends up shaded by its overall depth winner, ties to earlier objects.
+"""
+import numpy as np
+
+from . import _abi
+from .engine import Engine, MAX
+from .field import Field
+from .lighting import Lighting, SkyboxLighting
+from .matr import Diffuse
+from .postp import ToneMapping
+from .raster import TriangleRaster
+from .shader import Shader, ShaderGroup
+
+__all__ = ['Scene']
+
+_UNSUPPORTED = ('taa', 'ssr', 'ssao', 'fxaa', 'blooming')
+
+
+class _Obj:
+    def __init__(self, material, raster):
+        self.material = material
+        self.raster = raster
+        self.id_base = 0
+        self.nfaces = 0
+        self.fb = None
+
+
+class Scene:
+    def __init__(self, res_x=512, res_y=None, **options):
+        for k in _UNSUPPORTED:
+            if options.get(k, False):
+                raise NotImplementedError(f'Scene({k}=True): post-processing passes are outside this build')
+        self.options = options
+        self.engine = Engine(res_x, res_y, maxfaces=options.get('maxfaces', MAX))
+        self.res = self.engine.res
+        self.ibl = options.get('ibl', False)
+        self.tonemap = options.get('tonemap', True)
+        self.bgcolor = options.get('bgcolor', 0)
+        if not self.ibl:
+            self.lighting = Lighting()
+            # default lights (raster.py:89-93)
+            self.lighting.add_light(dir=[1, 2, 3], color=[0.9, 0.9, 0.9])
+            self.lighting.set_ambient_light([0.1, 0.1, 0.1])
+        else:
+            from .skybox import Skybox
+            sky = options.get('skybox')
+            if sky is None:
+                sky = Skybox.procedural()
+            self.lighting = SkyboxLighting(sky)
+        import torch
+        W, H = int(self.res[0]), int(self.res[1])
+        self.image = Field(torch.zeros((H, W, 3), dtype=torch.float32, device=self.engine.device))
+        self.default_material = Diffuse()
+        self.post_shaders = []
+        self.pre_shaders = []
+        self.materials = []
+        self.shaders = {}
+        self.objects = {}
+        if self.tonemap:
+            self.tonemap = ToneMapping(self.res)
+
+    def _ensure_material_shader(self, material):
+        if material in self.materials:
+            return
+        shader = Shader(self.image, self.lighting, material)
+        self.materials.append(material)
+        self.shaders[material] = ShaderGroup(self.pre_shaders + [shader] + self.post_shaders)
+
+    def add_object(self, object, material=None, raster=None):
+        """Adds a mesh; the raster is picked by duck typing (raster.py:112-146)."""
+        assert object not in self.objects
+        if material is None:
+            material = self.default_material
+        if raster is None:
+            if hasattr(object, 'get_nfaces') and not (hasattr(object, 'get_npolygon') and object.get_npolygon() == 2):
+                if not hasattr(self, 'triangle_raster'):
+                    self.triangle_raster = TriangleRaster(self.engine, **self.options)
+                raster = self.triangle_raster
+            elif hasattr(object, 'get_nfaces') or hasattr(object, 'get_npars') or hasattr(object, 'sample_volume'):
+                raise NotImplementedError(f'{type(object).__name__}: only triangle meshes are rasterised here')
+            else:
+                raise ValueError(f'cannot determine raster type of object: {object}')
+        self._ensure_material_shader(material)
+        self.objects[object] = _Obj(material, raster)
+
+    def set_camera(self, view, proj):
+        self.engine.set_camera(view, proj)
+
+    def input(self, gui):
+        raise NotImplementedError('GUI camera control is outside this build; use scene.set_camera(view, proj)')
+
+    def render(self, on_phase=None):
+        """clear -> coverage of every object -> shade per object (+bg, +ACES).
+
+        on_phase(name), if given, is called before the 'clear', 'raster' and
+        'shade' stages and once more with 'end' (used to place timing events)."""
+        eng = self.engine
+        mark = on_phase or (lambda name: None)
+        for s in self.pre_shaders + self.post_shaders:
+            s.clear_buffer()
+        mark('clear')
+        eng.clear_depth()
+        mark('raster')
+        order = []
+        for obj, info in self.objects.items():
+            r = info.raster
+            r.set_object(obj)
+            info.id_base, info.nfaces, info.fb = r.id_base, r.nfaces, r._fb
+            r.render_occup()
+            order.append((obj, info))
+        mark('shade')
+        self._shade_all(order)
+        mark('end')
+
+    def _shade_all(self, order):
+        tm = _abi.TONEMAP if self.tonemap else 0
+        if not order:
+            self._fill_background(tm)
+            return
+        first = True
+        for obj, info in order:
+            r = info.raster
+            r._fb, r._mesh, r.nfaces, r.id_base = info.fb, obj, info.nfaces, info.id_base
+            bg = _abi.BG if first else 0
+            r.render_color(self.shaders[info.material], extra_flags=bg | tm, bgcolor=self.bgcolor)
+            first = False
+
+    def _fill_background(self, tm):
+        import ctypes
+        if isinstance(self.lighting, SkyboxLighting):
+            # no faces: a shade launch over an empty id range paints the sky
+            r = TriangleRaster(self.engine, maxfaces=0)
+            r._fb = None
+            r.render_color(self.shaders.get(self.default_material) or
+                           ShaderGroup([Shader(self.image, self.lighting, self.default_material)]),
+                           extra_flags=_abi.BG | tm, bgcolor=self.bgcolor)
+            return
+        bg = (ctypes.c_float * 3)(*[float(x) for x in np.broadcast_to(np.asarray(self.bgcolor, dtype=np.float32), 3)])
+        _abi.check(self.engine._lib.tina_fill(self.engine._ctx, _abi.ptr(self.image.storage), ctypes.byref(bg),
+                                              tm, _abi.stream_handle()), self.engine._ctx)
+
+    @property
+    def img(self):
+        """The final (tone-mapped) image, a Field of shape (W, H, 3)."""
+        return self.image
+
+    def clear(self):
+        pass
+
+    def _fast_export_image(self, out):
+        """Flat RGB export, out[(y * W + x) * 3 + c] (raster.py:243-250)."""
+        out[:] = self.image.storage.detach().cpu().numpy().reshape(-1)
