@@ -155,6 +155,17 @@ def test_deterministic_repeat(gpu):
         assert np.array_equal(out[0][1], out[i][1])
 
 
+def test_plain_512_matches_golden_fixture(gpu):
+    """The HIP frame against the committed fixture tests/golden/plain_512.npz."""
+    import os
+    from conftest import ROOT
+    from taichi_three_amd import workloads as wl
+    g = np.load(os.path.join(ROOT, 'tests', 'golden', 'plain_512.npz'))
+    _, ids, depth, img = _render(wl.plain_512())
+    assert np.array_equal(ids, g['ids']) and np.array_equal(depth, g['depth'])
+    assert np.abs(img[192:320, 192:320] - g['img_crop']).max() <= COLOR_TOL_F32
+
+
 def test_errors_raise(gpu):
     import taichi_three_amd as tina
     eng = tina.Engine(64, 64, maxfaces=10)
