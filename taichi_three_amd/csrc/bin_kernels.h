@@ -1,0 +1,258 @@
+// bin_kernels.h -- binned, tiled coverage: the main render_occup path
+// (tina/core/triangle.py:88-131).
+//
+//   k_bin    one 1024-lane workgroup per kBinFaces consecutive faces.  Each face is
+//            set up (cull, clip, bbox) and counted into every kTile x kTile screen
+//            tile its bbox touches, in an LDS histogram.  One global atomic reserves
+//            the workgroup's block of bin entries; the workgroup writes, per tile, its
+//            (offset, count) chunk record and scatters its face ids into its block.
+//            Faces whose bbox touches more than kBinMaxTiles tiles go to the giant
+//            queue (k_giant, global atomics) instead.
+//   k_tile   one workgroup per screen tile.  The tile's chunks from all binning
+//            workgroups form one virtual list (prefix sum of their counts in LDS).
+//            The tile's keys are loaded into LDS once; each wave then pulls 64 faces
+//            at a time from the list, sets them up, clips their bboxes to the tile,
+//            and rasterises them with the wave-cooperative fragment expansion,
+//            reducing every covered pixel with an LDS 64-bit atomicMin.  Waves run
+//            independently (no workgroup barrier inside the loop).  The keys go
+//            back to HBM once, coalesced.
+//
+// No global atomic touches the key buffer on this path: the 64-bit atomic rate
+// to HBM-resident lines (about 27 G/s measured on MI355X) would bound the pass.
+#pragma once
+#include "raster_atomic.h"
+
+namespace tina {
+
+constexpr int kTileLog = 5;
+constexpr int kTile = 1 << kTileLog;  // 32 x 32 pixel tiles, 8 KB of keys in LDS
+constexpr int kBinBlock = 1024;       // lanes of a binning workgroup
+constexpr int kBinFaces = 4096;       // faces per binning workgroup (4 per lane)
+constexpr int kBinMaxTiles = 16;      // bbox tiles above which a face is rasterised as a giant
+constexpr int kMaxBinGroups = 8192;   // binning workgroups a k_tile workgroup can index
+
+struct BinCounters {
+  uint32_t cursor, giant, pad0, pad1;
+};
+
+// The tiles a face's bbox touches: x0 (12 bits), y0 (12 bits), x span - 1 (4 bits),
+// y span - 1 (4 bits).  0xFFFFFFFF: nothing to bin.
+__device__ __forceinline__ uint32_t pack_span(int tx0, int ty0, int nx, int ny) {
+  return (uint32_t)tx0 | ((uint32_t)ty0 << 12) | ((uint32_t)(nx - 1) << 24) | ((uint32_t)(ny - 1) << 28);
+}
+
+// Block-wide exclusive scan of n values v(i) into out(i), nthreads lanes at a time;
+// returns the total.  wsum: LDS scratch of nthreads/64 words; carry: one LDS word.
+template <int NT, class Get, class Put>
+__device__ __forceinline__ uint32_t block_exclusive_scan(int n, Get v, Put out, uint32_t* wsum, uint32_t* carry) {
+  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  if (tid == 0) *carry = 0;
+  __syncthreads();
+  for (int i0 = 0; i0 < n; i0 += NT) {
+    const int i = i0 + tid;
+    const uint32_t x = i < n ? v(i) : 0u;
+    const uint32_t incl = (uint32_t)wave_inclusive_scan((int)x, lane);
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t wofs = *carry, tot = 0;
+    for (int w = 0; w < NT / 64; w++) {
+      if (w < wave) wofs += wsum[w];
+      tot += wsum[w];
+    }
+    if (i < n) out(i, wofs + incl - x);
+    __syncthreads();
+    if (tid == 0) *carry += tot;
+    __syncthreads();
+  }
+  return *carry;
+}
+
+__global__ void __launch_bounds__(kBinBlock)
+k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, Cam cam, int tiles_x, int ntiles,
+      int ngroups, uint32_t* __restrict__ chunk_off, uint32_t* __restrict__ chunk_cnt, uint32_t* __restrict__ entries,
+      BigRec* __restrict__ giantq, BinCounters* __restrict__ ctr) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* cnt = lds;            // [ntiles] counts, then running offsets
+  uint32_t* span = lds + ntiles;  // [kBinFaces] per-face tile span
+  __shared__ uint32_t s_base, s_carry, s_wsum[kBinBlock / 64];
+  const int tid = threadIdx.x;
+  const int g = blockIdx.x;
+  const long f0 = (long)g * kBinFaces;
+  for (int t = tid; t < ntiles; t += kBinBlock) cnt[t] = 0;
+  __syncthreads();
+
+  for (int i = tid; i < kBinFaces; i += kBinBlock) {
+    const long f = f0 + i;
+    uint32_t sp = 0xFFFFFFFFu;
+    bool giant = false;
+    FaceSetup s;
+    if (f < T && setup_face(cam, verts + f * 9, flags, s) && s.topx >= s.botx && s.topy >= s.boty) {
+      const int tx0 = s.botx >> kTileLog, tx1 = s.topx >> kTileLog;
+      const int ty0 = s.boty >> kTileLog, ty1 = s.topy >> kTileLog;
+      const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1;
+      if (nx * ny > kBinMaxTiles) {
+        giant = true;
+      } else {
+        sp = pack_span(tx0, ty0, nx, ny);
+        for (int y = ty0; y <= ty1; y++)
+          for (int x = tx0; x <= tx1; x++) atomicAdd(&cnt[y * tiles_x + x], 1u);
+      }
+    }
+    span[i] = sp;
+    push_rec(giant, s, id_base + (uint32_t)f, giantq, &ctr->giant);
+  }
+  __syncthreads();
+
+  const uint32_t total = block_exclusive_scan<kBinBlock>(
+      ntiles, [&](int t) { return cnt[t]; },
+      [&](int t, uint32_t ofs) {
+        chunk_cnt[(size_t)t * ngroups + g] = cnt[t];
+        cnt[t] = ofs;
+      },
+      s_wsum, &s_carry);
+  if (tid == 0) s_base = atomicAdd(&ctr->cursor, total);
+  __syncthreads();
+  const uint32_t base = s_base;
+  for (int t = tid; t < ntiles; t += kBinBlock) chunk_off[(size_t)t * ngroups + g] = base + cnt[t];
+  __syncthreads();
+
+  // scatter the face ids (order within a tile is irrelevant: the reduction is a min)
+  for (int i = tid; i < kBinFaces; i += kBinBlock) {
+    const uint32_t sp = span[i];
+    if (sp == 0xFFFFFFFFu) continue;
+    const int tx0 = sp & 0xFFF, ty0 = (sp >> 12) & 0xFFF, nx = ((sp >> 24) & 0xF) + 1, ny = (sp >> 28) + 1;
+    for (int y = ty0; y < ty0 + ny; y++)
+      for (int x = tx0; x < tx0 + nx; x++) {
+        const uint32_t pos = atomicAdd(&cnt[y * tiles_x + x], 1u);
+        entries[base + pos] = (uint32_t)(f0 + i);
+      }
+  }
+}
+
+// Faces too large to bin: every lane of the grid takes a slice of each one and
+// reduces into the key buffer with global atomics.  Runs between k_bin and
+// k_tile (k_tile loads the keys after it and resets the queue).
+__global__ void __launch_bounds__(kBlock)
+k_giant(const BigRec* __restrict__ giantq, const BinCounters* __restrict__ ctr, Cam cam,
+        uint64_t* __restrict__ keys) {
+  const uint32_t ng = ctr->giant;
+  const long stride = (long)gridDim.x * kBlock;
+  for (uint32_t r = 0; r < ng; r++) {
+    const BigRec rec = giantq[r];
+    const long w = rec.s.topx - rec.s.botx + 1;
+    const long area = w * (long)(rec.s.topy - rec.s.boty + 1);
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < area; i += stride) {
+      const long dy = i / w;
+      cover_pixel(rec.s, cam, rec.s.botx + (int)(i - dy * w), rec.s.boty + (int)dy, rec.id, keys);
+    }
+  }
+}
+
+// Makes this lane's LDS writes visible to the rest of its wave before they are read.
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_tile(const float* __restrict__ verts, uint32_t id_base, uint32_t flags, Cam cam, int tiles_x, int ngroups,
+       const uint32_t* __restrict__ chunk_off, const uint32_t* __restrict__ chunk_cnt,
+       const uint32_t* __restrict__ entries, uint64_t* __restrict__ keys, BinCounters* __restrict__ ctr) {
+  extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
+  __shared__ unsigned long long tk[kTile * kTile];
+  __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock], R4[kBlock];
+  __shared__ int32_t Lend[kBlock];
+  __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
+  const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
+  const int t = blockIdx.x;
+  const int tx = t % tiles_x, ty = t / tiles_x;
+  const int ox = tx << kTileLog, oy = ty << kTileLog;
+  // every binning workgroup has finished: the next raster call starts from empty queues
+  if (t == 0 && tid == 0) {
+    ctr->cursor = 0;
+    ctr->giant = 0;
+  }
+  const uint32_t* cc = chunk_cnt + (size_t)t * ngroups;
+  const uint32_t total = block_exclusive_scan<kBlock>(
+      ngroups, [&](int c) { return cc[c]; }, [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);
+  if (total == 0) return;
+  if (tid == 0) {
+    pre[ngroups] = total;
+    s_next = 0;
+  }
+  const int tw = min(kTile, cam.W - ox), th = min(kTile, cam.H - oy);
+  for (int i = tid; i < kTile * kTile; i += kBlock) {
+    const int lx = i & (kTile - 1), ly = i >> kTileLog;
+    tk[i] = (lx < tw && ly < th) ? keys[(size_t)(oy + ly) * cam.W + ox + lx] : empty_key();
+  }
+  __syncthreads();
+
+  const uint32_t* co = chunk_off + (size_t)t * ngroups;
+  for (;;) {
+    uint32_t b0 = 0;
+    if (lane == 0) b0 = atomicAdd(&s_next, 64u);
+    b0 = __shfl(b0, 0, 64);
+    if (b0 >= total) break;
+    const uint32_t v = b0 + lane;
+    int area = 0;
+    if (v < total) {
+      int lo = 0, hi = ngroups;  // pre[lo] <= v < pre[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= v) lo = mid; else hi = mid;
+      }
+      const uint32_t fid = entries[co[lo] + (v - pre[lo])];
+      FaceSetup s;
+      if (setup_face(cam, verts + (size_t)fid * 9, flags, s)) {
+        const int bx0 = max(s.botx, ox), bx1 = min(s.topx, ox + tw - 1);
+        const int by0 = max(s.boty, oy), by1 = min(s.topy, oy + th - 1);
+        const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
+        if (w > 0 && h > 0) {
+          area = w * h;
+          R0[tid] = make_float4(s.bx, s.by, s.cx, s.cy);
+          R1[tid] = make_float4(s.bcnx, s.bcny, s.canx, s.cany);
+          R2[tid] = make_float4(s.ws0, s.ws1, s.ws2, s.Az);
+          R3[tid] = make_float4(s.Bz, s.Cz, __int_as_float(bx0), __int_as_float(by0));
+          R4[tid] = make_float4(__int_as_float(w), 1.0f / (float)w, __int_as_float((int)(id_base + fid)), 0.0f);
+        }
+      }
+    }
+    const int end = wave_inclusive_scan(area, lane);
+    Lend[tid] = end;
+    const int wtotal = __shfl(end, 63, 64);
+    wave_sync_lds();
+    for (int gI = lane; gI < wtotal; gI += 64) {
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (Lend[wbase + k + step - 1] <= gI) k += step;
+      const int j = wbase + k;
+      const int start = k > 0 ? Lend[j - 1] : 0;
+      const float4 r4 = R4[j];
+      const int wj = __float_as_int(r4.x);
+      const int li = gI - start;
+      const int dy = (int)(((float)li + 0.5f) * r4.y);  // li / wj, exact (li < 1024)
+      const int dx = li - dy * wj;
+      const float4 r0 = R0[j], r1 = R1[j], r2 = R2[j], r3 = R3[j];
+      FaceSetup q;
+      q.bx = r0.x; q.by = r0.y; q.cx = r0.z; q.cy = r0.w;
+      q.bcnx = r1.x; q.bcny = r1.y; q.canx = r1.z; q.cany = r1.w;
+      q.ws0 = r2.x; q.ws1 = r2.y; q.ws2 = r2.z; q.Az = r2.w;
+      q.Bz = r3.x; q.Cz = r3.y;
+      const int px = __float_as_int(r3.z) + dx, py = __float_as_int(r3.w) + dy;
+      float w0, w1, w2;
+      if (pixel_cover_fast(q, (float)px + cam.bias[0], (float)py + cam.bias[1], w0, w1, w2)) {
+        const uint64_t key = pack_key(pixel_depth(q, w0, w1, w2), (uint32_t)__float_as_int(r4.z));
+        atomicMin(&tk[((py - oy) << kTileLog) + (px - ox)], (unsigned long long)key);
+      }
+    }
+    wave_sync_lds();
+  }
+  __syncthreads();
+  for (int i = tid; i < kTile * kTile; i += kBlock) {
+    const int lx = i & (kTile - 1), ly = i >> kTileLog;
+    if (lx < tw && ly < th) keys[(size_t)(oy + ly) * cam.W + ox + lx] = tk[i];
+  }
+}
+
+}  // namespace tina

@@ -29,6 +29,7 @@ __device__ __forceinline__ V3 vcross(V3 a, V3 b) {
 // reflect(I, N) = I - 2 * N.dot(I) * N   (tina/common.py:197-199)
 __device__ __forceinline__ V3 vreflect(V3 I, V3 N) { const float d = 2.0f * vdot(N, I); return vsub(I, vsmul(d, N)); }
 __device__ __forceinline__ V3 ld3(const float* __restrict__ p) { return v3(p[0], p[1], p[2]); }
+__device__ __forceinline__ V3 ld3(gf32* __restrict__ p) { return v3(p[0], p[1], p[2]); }
 
 // mapply_pos (tina/common.py:180-183)
 __device__ __forceinline__ V3 mapply_pos(const float* __restrict__ M, V3 p, float* wout = nullptr) {
@@ -41,17 +42,18 @@ __device__ __forceinline__ V3 mapply_pos(const float* __restrict__ M, V3 p, floa
 // bilerp (tina/common.py:140-149) over a [w][h][c] array; the indices are
 // clamped to the array (the reference reads one texel past the edge with
 // weight 0, or out of bounds for coordinates outside [0, shape-1]).
-__device__ __forceinline__ V3 bilerp(const float* __restrict__ img, int w, int h, int c, int ch,
+template <class PTR>
+__device__ __forceinline__ V3 bilerp(PTR __restrict__ img, int w, int h, int c, int ch,
                                      float px, float py) {
   const int ix = f2i_sat(floorf(px)), iy = f2i_sat(floorf(py));
   const float x0 = px - (float)ix, x1 = py - (float)iy;
   const float y0 = 1.0f - x0, y1 = 1.0f - x1;
   const int xa = min(max(ix, 0), w - 1), xb = min(max(ix + 1, 0), w - 1);
   const int ya = min(max(iy, 0), h - 1), yb = min(max(iy + 1, 0), h - 1);
-  const float* t00 = img + ((long)xa * h + ya) * c + ch;
-  const float* t01 = img + ((long)xa * h + yb) * c + ch;
-  const float* t10 = img + ((long)xb * h + ya) * c + ch;
-  const float* t11 = img + ((long)xb * h + yb) * c + ch;
+  const auto t00 = img + ((long)xa * h + ya) * c + ch;
+  const auto t01 = img + ((long)xa * h + yb) * c + ch;
+  const auto t10 = img + ((long)xb * h + ya) * c + ch;
+  const auto t11 = img + ((long)xb * h + yb) * c + ch;
   V3 f00, f01, f10, f11;
   if (c >= 3) { f00 = ld3(t00); f01 = ld3(t01); f10 = ld3(t10); f11 = ld3(t11); }
   else { f00 = vs(t00[0]); f01 = vs(t01[0]); f10 = vs(t10[0]); f11 = vs(t11[0]); }
@@ -63,7 +65,7 @@ __device__ __forceinline__ V3 bilerp(const float* __restrict__ img, int w, int h
 }
 
 // Skybox.sample for an equirect map (tina/skybox.py:23-28, 138-145)
-__device__ __forceinline__ V3 skybox_sample(const float* __restrict__ img, int w, int h, V3 dir) {
+__device__ __forceinline__ V3 skybox_sample(gf32* __restrict__ img, int w, int h, V3 dir) {
   const float p = atan2f(-dir.z, dir.x) / 6.2831855f;
   const float cx = p - floorf(p);
   const float cy = dir.y * 0.5f + 0.5f;
@@ -117,8 +119,8 @@ __device__ __forceinline__ V3 cooktorrance_ibl(const tina_ibl& ibl, float roughn
   for (int id = 0; id < 6; id++) {
     const float lo = ibl.rough_lo[id], hi = ibl.rough_hi[id], st = ibl.rough_step[id];
     if (lo <= roughness && roughness <= hi) {
-      const V3 w1 = skybox_sample((const float*)ibl.spec[id], ibl.spec_w[id], ibl.spec_h[id], rdir);
-      const V3 w2 = skybox_sample((const float*)ibl.spec[id + 1], ibl.spec_w[id + 1], ibl.spec_h[id + 1], rdir);
+      const V3 w1 = skybox_sample(gptr(ibl.spec[id]), ibl.spec_w[id], ibl.spec_h[id], rdir);
+      const V3 w2 = skybox_sample(gptr(ibl.spec[id + 1]), ibl.spec_w[id + 1], ibl.spec_h[id + 1], rdir);
       const float fac2 = (hi - roughness) / st;
       wei = vadd(vsmul(1.0f - fac2, w2), vsmul(fac2, w1));  // lerp(fac2, wei2, wei1)
     }
@@ -131,6 +133,44 @@ __device__ __forceinline__ V3 cooktorrance_ibl(const tina_ibl& ibl, float roughn
 }
 
 enum { M_BRDF = 0, M_AMB = 1, M_EMI = 2, M_IBL = 3 };
+
+// The material, light and IBL tables arrive by value in the kernel-argument
+// segment.  Indexing them with a run-time index would make the compiler copy the
+// whole 2 KB argument block to per-lane scratch, so every access goes through a
+// switch: each case reads the kernarg segment at a fixed offset (the index is
+// uniform across the launch, so the switch is a scalar branch).
+#define TINA_CASE16(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
+
+__device__ __forceinline__ tina_param param_at(const tina_material& m, int i) {
+  switch (i) {
+#define TINA_P(k) case k: if (k < TINA_MAX_PARAMS) return m.params[k < TINA_MAX_PARAMS ? k : 0]; break;
+    TINA_CASE16(TINA_P)
+#undef TINA_P
+  }
+  return m.params[0];
+}
+
+__device__ __forceinline__ tina_op op_at(const tina_material& m, int i) {
+  switch (i) {
+#define TINA_O(k) case k: return m.ops[k];
+    TINA_CASE16(TINA_O)
+#undef TINA_O
+  }
+  return m.ops[0];
+}
+
+struct Light { V3 xyz; float w; V3 color; };
+__device__ __forceinline__ Light light_at(const tina_lights& lt, int l) {
+  Light r;
+  switch (l) {
+#define TINA_L(k) case k: r.xyz = v3(lt.dirs[k][0], lt.dirs[k][1], lt.dirs[k][2]); r.w = lt.dirs[k][3]; \
+      r.color = v3(lt.colors[k][0], lt.colors[k][1], lt.colors[k][2]); break;
+    TINA_CASE16(TINA_L)
+#undef TINA_L
+    default: r.xyz = vs(0.0f); r.w = 0.0f; r.color = vs(0.0f); break;
+  }
+  return r;
+}
 
 // Parameter slot values of one pixel, in registers.
 struct Slots {
@@ -152,13 +192,14 @@ __device__ __forceinline__ V3 run_material(const tina_material& m, const Slots& 
   V3 s0 = vs(0.0f), s1 = vs(0.0f), s2 = vs(0.0f), s3 = vs(0.0f);
   const float inv_pi = (float)(1.0 / 3.141592653589793);
   for (int i = 0; i < m.nops; i++) {
-    const int op = m.ops[i].op, a = m.ops[i].a, b = m.ops[i].b;
+    const tina_op o = op_at(m, i);
+    const int op = o.op, a = o.a, b = o.b;
     if (op <= TINA_OP_EMISSION) {
       V3 v = vs(0.0f);
       if (op == TINA_OP_LAMBERT) {
         if (mode == M_BRDF) v = vs(inv_pi);
         else if (mode == M_AMB) v = vs(1.0f);
-        else if (mode == M_IBL) v = skybox_sample((const float*)ibl.diff, ibl.diff_w, ibl.diff_h, n);
+        else if (mode == M_IBL) v = skybox_sample(gptr(ibl.diff), ibl.diff_w, ibl.diff_h, n);
       } else if (op == TINA_OP_PHONG || op == TINA_OP_BLINNPHONG) {
         if (mode == M_BRDF)
           v = op == TINA_OP_PHONG ? phong(pv.get(a).x, n, L, V) : blinnphong(pv.get(a).x, n, L, V);
@@ -200,14 +241,15 @@ __device__ __forceinline__ V3 lighting(const tina_material& m, const Slots& pv, 
   }
   if (flags & TINA_NOLIGHT) return res;
   for (int l = 0; l < lt.n; l++) {
-    const V3 lxyz = v3(lt.dirs[l][0], lt.dirs[l][1], lt.dirs[l][2]);
-    const float lw = lt.dirs[l][3];
+    const Light li = light_at(lt, l);
+    const V3 lxyz = li.xyz;
+    const float lw = li.w;
     V3 ldir = vsub(lxyz, v3(pos.x * lw, pos.y * lw, pos.z * lw));
     const float dist = sqrtf(vdot(ldir, ldir));
     ldir = vdivs(ldir, dist);
     const float cos_i = vdot(n, ldir);
     if (cos_i > 0.0f) {
-      V3 lcolor = v3(lt.colors[l][0], lt.colors[l][1], lt.colors[l][2]);
+      V3 lcolor = li.color;
       lcolor = vdivs(lcolor, dist * dist);
       const V3 mcolor = run_material(m, pv, M_BRDF, n, ldir, V, ibl);
       res = vadd(res, vmul(vsmul(cos_i, lcolor), mcolor));
@@ -228,12 +270,12 @@ __device__ __forceinline__ void eval_params(const tina_material& m, uint32_t f, 
 #pragma unroll
   for (int i = 0; i < TINA_MAX_PARAMS; i++) {
     if (i < m.nparams) {
-      const tina_param& p = m.params[i];
+      const tina_param p = param_at(m, i);
       V3 v = vs(0.0f);
       switch (p.kind) {
         case TINA_P_CONST: v = p.dim == 1 ? vs(p.value[0]) : v3(p.value[0], p.value[1], p.value[2]); break;
         case TINA_P_PARAM: {
-          const float* q = (const float*)p.ptr;
+          gf32* q = gptr(p.ptr);
           v = p.dim == 1 ? vs(q[0]) : v3(q[0], q[1], q[2]);
           break;
         }
@@ -243,12 +285,12 @@ __device__ __forceinline__ void eval_params(const tina_material& m, uint32_t f, 
         case TINA_P_TEXCOORD: v = tc; break;
         case TINA_P_TEXTURE: {
           const V3 t = pv.get(p.a);
-          v = bilerp((const float*)p.ptr, p.tex_w, p.tex_h, p.tex_c, 0, t.x * (float)(p.tex_w - 1),
+          v = bilerp(gptr(p.ptr), p.tex_w, p.tex_h, p.tex_c, 0, t.x * (float)(p.tex_w - 1),
                      t.y * (float)(p.tex_h - 1));
           break;
         }
         case TINA_P_FACE: {
-          const float* q = (const float*)p.ptr;
+          gf32* q = gptr(p.ptr);
           v = p.dim == 1 ? vs(q[f]) : v3(q[3 * (size_t)f], q[3 * (size_t)f + 1], q[3 * (size_t)f + 2]);
           break;
         }
@@ -284,7 +326,7 @@ struct Frag {
 };
 
 __device__ __forceinline__ void fragment(const ShadeArgs& a, uint32_t f, int px, int py, Frag& fr) {
-  const float* vp = (const float*)a.mesh.verts + 9 * (size_t)f;
+  gf32* vp = gptr(a.mesh.verts) + 9 * (size_t)f;
   const V3 A = ld3(vp), B = ld3(vp + 3), C = ld3(vp + 6);
   float wA, wB, wC;
   const V3 Av = mapply_pos(a.cam.W2V, A, &wA);
@@ -310,7 +352,7 @@ __device__ __forceinline__ void fragment(const ShadeArgs& a, uint32_t f, int px,
   fr.pos = vadd(vadd(vsmul(wx, A), vsmul(wy, B)), vsmul(wz, C));
   V3 nrm;
   if ((a.flags & TINA_SMOOTH) && a.mesh.norms) {
-    const float* np = (const float*)a.mesh.norms + 9 * (size_t)f;
+    gf32* np = gptr(a.mesh.norms) + 9 * (size_t)f;
     nrm = vadd(vadd(vsmul(wx, ld3(np)), vsmul(wy, ld3(np + 3))), vsmul(wz, ld3(np + 6)));
   } else {
     nrm = vcross(vsub(B, A), vsub(C, A));
@@ -318,12 +360,12 @@ __device__ __forceinline__ void fragment(const ShadeArgs& a, uint32_t f, int px,
   fr.nrm = vnormalized(nrm);
   fr.tc = vs(0.0f);
   if ((a.flags & TINA_TEX) && a.mesh.coors) {
-    const float* q = (const float*)a.mesh.coors + 6 * (size_t)f;
+    gf32* q = gptr(a.mesh.coors) + 6 * (size_t)f;
     fr.tc = v3(((wx * q[0]) + (wy * q[2])) + (wz * q[4]), ((wx * q[1]) + (wy * q[3])) + (wz * q[5]), 0.0f);
   }
   fr.color = vs(1.0f);
   if ((a.flags & TINA_VCOLOR) && a.mesh.colors) {
-    const float* cp = (const float*)a.mesh.colors + 9 * (size_t)f;
+    gf32* cp = gptr(a.mesh.colors) + 9 * (size_t)f;
     fr.color = vadd(vadd(vsmul(wx, ld3(cp)), vsmul(wy, ld3(cp + 3))), vsmul(wz, ld3(cp + 6)));
   }
 }
@@ -337,21 +379,151 @@ __device__ __forceinline__ V3 view_ray(const Cam& cam, int px, int py) {
   return vnormalized(vsub(ro1, ro));
 }
 
+// ---- specialised material programs ------------------------------------------
+//
+// The presets of tina/matr/material.py:684-706 lower to four fixed programs:
+//   Diffuse  [LAMBERT, SCALE a]
+//   Classic  [LAMBERT, SCALE a, PHONG m, MIX s]
+//   Blinn    [LAMBERT, SCALE a, BLINNPHONG m, MIX s]
+//   PBR      [LAMBERT, SCALE a, COOKTORRANCE r f, MIX c]
+// For these the kernel is compiled with the program fixed, so no parameter
+// array and no interpreter stack are needed; the arithmetic (and its order) is
+// the one run_material performs for the same program.
+enum Prog { PROG_GENERIC = 0, PROG_DIFFUSE = 1, PROG_CLASSIC = 2, PROG_BLINN = 3, PROG_PBR = 4 };
+
+// Value of a non-derived parameter slot (everything but FRESNEL and TEXTURE).
+__device__ __forceinline__ V3 eval_basic(const tina_param& p, uint32_t f, const Frag& fr) {
+  switch (p.kind) {
+    case TINA_P_CONST: return p.dim == 1 ? vs(p.value[0]) : v3(p.value[0], p.value[1], p.value[2]);
+    case TINA_P_PARAM: {
+      gf32* q = gptr(p.ptr);
+      return p.dim == 1 ? vs(q[0]) : v3(q[0], q[1], q[2]);
+    }
+    case TINA_P_COLOR: return fr.color;
+    case TINA_P_POS: return fr.pos;
+    case TINA_P_NORMAL: return fr.nrm;
+    case TINA_P_TEXCOORD: return fr.tc;
+    case TINA_P_FACE: {
+      gf32* q = gptr(p.ptr);
+      return p.dim == 1 ? vs(q[f]) : v3(q[3 * (size_t)f], q[3 * (size_t)f + 1], q[3 * (size_t)f + 2]);
+    }
+    default: return vs(0.0f);
+  }
+}
+
+__device__ __forceinline__ V3 eval_leaf(const tina_material& m, int i, uint32_t f, const Frag& fr) {
+  const tina_param p = param_at(m, i);
+  if (p.kind == TINA_P_TEXTURE) {
+    const V3 t = eval_basic(param_at(m, p.a), f, fr);
+    return bilerp(gptr(p.ptr), p.tex_w, p.tex_h, p.tex_c, 0, t.x * (float)(p.tex_w - 1),
+                  t.y * (float)(p.tex_h - 1));
+  }
+  return eval_basic(p, f, fr);
+}
+
+__device__ __forceinline__ V3 eval_slot(const tina_material& m, int i, uint32_t f, const Frag& fr) {
+  const tina_param p = param_at(m, i);
+  if (p.kind == TINA_P_FRESNEL) {  // metallic * albedo + (1 - metallic) * 0.16 * specular**2
+    const V3 mt = eval_leaf(m, p.a, f, fr), al = eval_leaf(m, p.b, f, fr), sp = eval_leaf(m, p.c, f, fr);
+    return vadd(vmul(mt, al), vmul(vsmul(0.16f, vsub(vs(1.0f), mt)), vmul(sp, sp)));
+  }
+  return eval_leaf(m, i, f, fr);
+}
+
+template <int PROG, bool IBL>
+__device__ __forceinline__ V3 shade_fixed(const ShadeArgs& a, uint32_t f, const Frag& fr, V3 V) {
+  const tina_material& m = a.mat;
+  const float inv_pi = (float)(1.0 / 3.141592653589793);
+  const V3 n = fr.nrm;
+  const V3 alb = eval_slot(m, m.ops[1].a, f, fr);
+  V3 fac = vs(0.0f), f0 = vs(0.0f);
+  float lobe_p = 0.0f;
+  if constexpr (PROG != PROG_DIFFUSE) {
+    fac = eval_slot(m, m.ops[3].a, f, fr);
+    if constexpr (PROG == PROG_PBR) {
+      lobe_p = eval_slot(m, m.ops[2].a, f, fr).x;  // roughness
+      f0 = m.ops[2].b == m.ops[3].a ? fac : eval_slot(m, m.ops[2].b, f, fr);
+    } else {
+      lobe_p = eval_slot(m, m.ops[2].a, f, fr).x;  // shineness
+    }
+  }
+  const V3 omf = vsub(vs(1.0f), fac);
+  V3 res;
+  if constexpr (IBL) {
+    // sample_ibl: Scale -> wei * fac, Lambert -> diff.sample(nrm), Mix -> (1-f)*w1 + f*w2
+    const V3 d = vmul(alb, skybox_sample(gptr(a.ibl.diff), a.ibl.diff_w, a.ibl.diff_h, n));
+    if constexpr (PROG == PROG_DIFFUSE) res = d;
+    else res = vadd(vmul(omf, d), vmul(fac, cooktorrance_ibl(a.ibl, lobe_p, f0, V, n)));
+  } else {
+    // emission (0) + ambient_color * ambient(): Scale -> fac * 1, Mix -> (1-f)*w1 + f*1
+    const V3 amb = v3(a.lights.ambient[0], a.lights.ambient[1], a.lights.ambient[2]);
+    V3 ma = vmul(alb, vs(1.0f));
+    if constexpr (PROG != PROG_DIFFUSE) ma = vadd(vmul(omf, ma), vmul(fac, vs(1.0f)));
+    res = vadd(vs(0.0f), vmul(amb, ma));
+  }
+  if (a.flags & TINA_NOLIGHT) return res;
+  const V3 diff_brdf = vmul(alb, vs(inv_pi));
+  for (int l = 0; l < a.lights.n; l++) {
+    const Light li = light_at(a.lights, l);
+    const V3 lxyz = li.xyz;
+    const float lw = li.w;
+    V3 ldir = vsub(lxyz, v3(fr.pos.x * lw, fr.pos.y * lw, fr.pos.z * lw));
+    const float dist = sqrtf(vdot(ldir, ldir));
+    ldir = vdivs(ldir, dist);
+    const float cos_i = vdot(n, ldir);
+    if (cos_i > 0.0f) {
+      V3 lcolor = li.color;
+      lcolor = vdivs(lcolor, dist * dist);
+      V3 mcolor;
+      if constexpr (PROG == PROG_DIFFUSE) {
+        mcolor = diff_brdf;
+      } else {
+        V3 spec;
+        if constexpr (PROG == PROG_CLASSIC) spec = phong(lobe_p, n, ldir, V);
+        else if constexpr (PROG == PROG_BLINN) spec = blinnphong(lobe_p, n, ldir, V);
+        else spec = cooktorrance(lobe_p, f0, n, ldir, V);
+        mcolor = vadd(vmul(omf, diff_brdf), vmul(fac, spec));
+      }
+      res = vadd(res, vmul(vsmul(cos_i, lcolor), mcolor));
+    }
+  }
+  return res;
+}
+
+template <int PROG, bool IBL>
 __device__ __forceinline__ V3 shade_pixel(const ShadeArgs& a, uint32_t f, int px, int py) {
   Frag fr;
   fragment(a, f, px, py, fr);
   const V3 viewdir = vneg(view_ray(a.cam, px, py));
-  Slots pv;
-  eval_params(a.mat, f, fr.pos, fr.nrm, fr.tc, fr.color, pv);
-  V3 res = lighting(a.mat, pv, a.lights, a.ibl, a.flags, fr.pos, fr.nrm, viewdir);
+  V3 res;
+  if constexpr (PROG == PROG_GENERIC) {
+    Slots pv;
+    eval_params(a.mat, f, fr.pos, fr.nrm, fr.tc, fr.color, pv);
+    res = lighting(a.mat, pv, a.lights, a.ibl, IBL ? (a.flags | TINA_IBL) : (a.flags & ~TINA_IBL), fr.pos,
+                   fr.nrm, viewdir);
+  } else {
+    res = shade_fixed<PROG, IBL>(a, f, fr, viewdir);
+  }
   if (a.flags & TINA_TONEMAP) res = aces(res);
   return res;
+}
+
+// Host-side pattern match of a lowered program onto one of the fixed kernels.
+inline int classify_program(const tina_material& m, bool ibl) {
+  const tina_op* o = m.ops;
+  if (m.nops == 2 && o[0].op == TINA_OP_LAMBERT && o[1].op == TINA_OP_SCALE) return PROG_DIFFUSE;
+  if (m.nops == 4 && o[0].op == TINA_OP_LAMBERT && o[1].op == TINA_OP_SCALE && o[3].op == TINA_OP_MIX) {
+    if (o[2].op == TINA_OP_COOKTORRANCE) return PROG_PBR;
+    if (!ibl && o[2].op == TINA_OP_PHONG) return PROG_CLASSIC;
+    if (!ibl && o[2].op == TINA_OP_BLINNPHONG) return PROG_BLINN;
+  }
+  return PROG_GENERIC;
 }
 
 __device__ __forceinline__ V3 background(const ShadeArgs& a, int px, int py) {
   V3 c = v3(a.bg[0], a.bg[1], a.bg[2]);
   if (a.flags & TINA_IBL)  // Engine.render_background (engine.py:41-49)
-    c = skybox_sample((const float*)a.ibl.spec[0], a.ibl.spec_w[0], a.ibl.spec_h[0], view_ray(a.cam, px, py));
+    c = skybox_sample(gptr(a.ibl.spec[0]), a.ibl.spec_w[0], a.ibl.spec_h[0], view_ray(a.cam, px, py));
   if (a.flags & TINA_TONEMAP) c = aces(c);
   return c;
 }

@@ -1,0 +1,127 @@
+// shade_kernels.h -- deferred shading and the image passes.
+//
+//   k_shade<PROG, IBL>  render_color + Shader + Lighting + material (+ ACES, + bg)
+//                       (tina/core/triangle.py:133-153, shader.py:112-131,
+//                       lighting.py:84-98): each workgroup owns kShadePix pixels;
+//                       it reads their keys, paints the empty ones with the
+//                       background, compacts the covered ones into an LDS list
+//                       with a wave ballot, then shades that list densely.
+//   k_fill              Scene.image.fill(bgcolor) (tina/scene/raster.py:176)
+//   k_aces              ToneMapping.apply (tina/postp/tonemap.py:9-12)
+//   k_resolve           unpacks keys into Engine.depth / face ids
+#pragma once
+#include "bin_kernels.h"
+#include "shade_dev.h"
+
+namespace tina {
+
+constexpr int kShadePix = 1024;  // pixels per shade workgroup
+
+__device__ __forceinline__ void store_rgb(float* __restrict__ img, long P, V3 c) {
+  img[3 * P] = c.x;
+  img[3 * P + 1] = c.y;
+  img[3 * P + 2] = c.z;
+}
+
+template <int PROG, bool IBL>
+__global__ void __launch_bounds__(kBlock)
+k_shade(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __restrict__ img) {
+  // Read the 2 KB argument block in place (it is the first kernel argument, at
+  // offset 0 of the kernarg segment): naming the by-value parameter would copy it
+  // to per-lane scratch.
+  (void)a_by_value;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const ShadeArgs& a = a_by_value;  // the host pass only needs the signature
+#endif
+  __shared__ uint32_t s_pix[kShadePix];
+  __shared__ uint32_t s_face[kShadePix];
+  __shared__ uint32_t s_n;
+  const int tid = threadIdx.x;
+  const long npix = (long)a.cam.W * a.cam.H;
+  const long base = (long)blockIdx.x * kShadePix;
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  const int lane = __lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int r = 0; r < kShadePix / kBlock; r++) {
+    const long P = base + r * kBlock + tid;
+    bool take = false;
+    uint32_t f = 0;
+    if (P < npix) {
+      const uint64_t k = keys[P];
+      if (key_depth(k) >= TINA_MAXDEPTH) {
+        if (a.flags & TINA_BG) store_rgb(img, P, background(a, (int)(P % a.cam.W), (int)(P / a.cam.W)));
+      } else {
+        f = (uint32_t)k - a.mesh.id_base;
+        take = f < (uint32_t)a.mesh.nfaces;
+      }
+    }
+    // wave-ballot compaction of the covered pixels into the LDS list
+    const unsigned long long mask = __ballot(take);
+    uint32_t off = 0;
+    if (lane == 0 && mask) off = atomicAdd(&s_n, (uint32_t)__popcll(mask));
+    off = __shfl(off, 0);
+    if (take) {
+      const uint32_t j = off + (uint32_t)__popcll(mask & lt_mask);
+      s_pix[j] = (uint32_t)(P - base);
+      s_face[j] = f;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = s_n;
+  for (uint32_t j = tid; j < n; j += kBlock) {
+    const long P = base + s_pix[j];
+    const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
+    store_rgb(img, P, shade_pixel<PROG, IBL>(a, s_face[j], px, py));
+  }
+}
+
+template <int PROG, bool IBL>
+void launch_shade(unsigned grid, hipStream_t s, const ShadeArgs& a, const uint64_t* keys, float* img) {
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<PROG, IBL>), dim3(grid), dim3(kBlock), 0, s, a, keys, img);
+}
+
+inline void dispatch_shade(int prog, bool ibl, unsigned grid, hipStream_t s, const ShadeArgs& a,
+                           const uint64_t* keys, float* img) {
+  if (ibl) {
+    switch (prog) {
+      case PROG_DIFFUSE: launch_shade<PROG_DIFFUSE, true>(grid, s, a, keys, img); break;
+      case PROG_PBR: launch_shade<PROG_PBR, true>(grid, s, a, keys, img); break;
+      default: launch_shade<PROG_GENERIC, true>(grid, s, a, keys, img); break;
+    }
+  } else {
+    switch (prog) {
+      case PROG_DIFFUSE: launch_shade<PROG_DIFFUSE, false>(grid, s, a, keys, img); break;
+      case PROG_CLASSIC: launch_shade<PROG_CLASSIC, false>(grid, s, a, keys, img); break;
+      case PROG_BLINN: launch_shade<PROG_BLINN, false>(grid, s, a, keys, img); break;
+      case PROG_PBR: launch_shade<PROG_PBR, false>(grid, s, a, keys, img); break;
+      default: launch_shade<PROG_GENERIC, false>(grid, s, a, keys, img); break;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill(const ShadeArgs a, float* __restrict__ img) {
+  const long npix = (long)a.cam.W * a.cam.H;
+  for (long P = (long)blockIdx.x * kBlock + threadIdx.x; P < npix; P += (long)gridDim.x * kBlock)
+    store_rgb(img, P, background(a, (int)(P % a.cam.W), (int)(P / a.cam.W)));
+}
+
+__global__ void __launch_bounds__(kBlock) k_aces(float* __restrict__ img, long n) {
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+    img[i] = aces1(img[i]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_resolve(const uint64_t* __restrict__ keys, int32_t* __restrict__ id, int32_t* __restrict__ depth, long n) {
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const uint64_t k = keys[i];
+    const int32_t d = key_depth(k);
+    if (depth) depth[i] = d;
+    if (id) id[i] = d >= TINA_MAXDEPTH ? -1 : (int32_t)(uint32_t)k;
+  }
+}
+
+}  // namespace tina

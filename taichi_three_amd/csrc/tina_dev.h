@@ -32,6 +32,13 @@ __device__ __forceinline__ int32_t key_depth(uint64_t k) {
   return (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
 }
 
+// Buffers that reach a kernel as integers inside an argument struct (mesh
+// attributes, textures, IBL maps) are global memory: addressing them through a
+// global-address-space pointer gives global_load instructions instead of flat
+// loads, which would also occupy the LDS counter.
+typedef const __attribute__((address_space(1))) float gf32;
+__device__ __forceinline__ gf32* gptr(uint64_t p) { return (gf32*)p; }
+
 // Camera passed by value as a kernel argument (lives in the kernarg segment).
 struct Cam {
   float W2V[16];
@@ -115,6 +122,39 @@ __device__ __forceinline__ bool pixel_weights(const FaceSetup& s, float posx, fl
   const float t = (1.0f - w_bc) - w_ca;
   const float x = w_bc * s.ws0, y = w_ca * s.ws1, z = t * s.ws2;
   const float sum = (x + y) + z;
+  w0 = x / sum; w1 = y / sum; w2 = z / sum;
+  return (w0 >= 0.0f) & (w1 >= 0.0f) & (w2 >= 0.0f);
+}
+
+// The same test and weights, dividing only when the pixel is covered.
+//
+// wei_i = x_i / sum >= 0 is decided from signs when that is exact: for finite x_i
+// and a finite sum with 2**-126 <= |sum| <= 2**100, x_i / sum can only round to
+// zero if |x_i| < |sum| * 2**-126, so whenever every nonzero |x_i| is at least
+// that, "x_i / sum >= 0" equals "x_i == 0 or x_i has the sign of sum".  Anything
+// else (zero or tiny sums, NaN, infinities, tiny x_i) takes the division path,
+// which is the reference's expression itself.  Returns the coverage; when covered
+// w0..w2 hold the divided weights, bit-identical to pixel_weights.
+__device__ __forceinline__ bool pixel_cover_fast(const FaceSetup& s, float posx, float posy,
+                                                 float& w0, float& w1, float& w2) {
+  const float w_bc = (posx - s.bx) * s.bcny - (posy - s.by) * s.bcnx;
+  const float w_ca = (posx - s.cx) * s.cany - (posy - s.cy) * s.canx;
+  const float t = (1.0f - w_bc) - w_ca;
+  const float x = w_bc * s.ws0, y = w_ca * s.ws1, z = t * s.ws2;
+  const float sum = (x + y) + z;
+  const float as = fabsf(sum);
+  const float lo = as * 1.1754944e-38f;  // |sum| * 2**-126, exact (power of two, no underflow below)
+  const bool sane = (as >= 1.1754944e-38f) & (as <= 1.2676506e30f) &
+                    ((x == 0.0f) | (fabsf(x) >= lo)) & ((y == 0.0f) | (fabsf(y) >= lo)) &
+                    ((z == 0.0f) | (fabsf(z) >= lo)) & (fabsf(x) <= 1.2676506e30f) &
+                    (fabsf(y) <= 1.2676506e30f) & (fabsf(z) <= 1.2676506e30f);
+  if (sane) {
+    const bool pos = sum > 0.0f;
+    const bool cx_ = (x == 0.0f) | ((x > 0.0f) == pos);
+    const bool cy_ = (y == 0.0f) | ((y > 0.0f) == pos);
+    const bool cz_ = (z == 0.0f) | ((z > 0.0f) == pos);
+    if (!(cx_ & cy_ & cz_)) return false;
+  }
   w0 = x / sum; w1 = y / sum; w2 = z / sum;
   return (w0 >= 0.0f) & (w1 >= 0.0f) & (w2 >= 0.0f);
 }

@@ -1,19 +1,10 @@
-// tina_kernels.hip -- the MI355X kernels of the rasterise-and-shade path and the
-// C ABI declared in include/tina_hip.h.
+// tina_kernels.hip -- the C ABI of include/tina_hip.h over the MI355X kernels of
+// raster_kernels.h (coverage) and shade_kernels.h (deferred shading, image passes).
 //
-//   k_clear         Engine.clear_depth + occup = -1 (engine.py:67-70, triangle.py:90-91)
-//   k_raster_small  render_occup (triangle.py:92-131), one lane per face: setup,
-//                   cull, clip, bbox; faces with a bbox of <= SMALL_AREA pixels are
-//                   scanned by their own lane with a 64-bit depth|id atomicMin;
-//                   larger ones are appended (wave-ballot compaction, one atomic
-//                   per wave) to the big or giant queue.
-//   k_raster_big    one workgroup per queued face, pixel-parallel over its bbox.
-//   k_raster_giant  faces whose bbox exceeds GIANT_AREA: every workgroup takes a
-//                   slice of every giant face, so one huge face spreads over the chip.
-//   k_shade         render_color + Shader + Lighting + material (+ ACES, + bg):
-//                   per workgroup 1024 pixels, covered ones compacted with a wave
-//                   ballot into an LDS list, then shaded densely.
-//   k_fill, k_aces, k_resolve   image fill, ToneMapping.apply, key unpacking.
+// Every entry point validates its arguments on the host before it enqueues
+// anything, returns a status code, and records a message for tina_last_error().
+// The per-frame calls only launch kernels on the caller's stream: no allocation,
+// no synchronisation, no memset (so a caller may capture them in a HIP graph).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -21,238 +12,23 @@
 #include <string>
 #include <new>
 
-#include "shade_dev.h"
+#include "bin_kernels.h"
+#include "shade_kernels.h"
 
 using namespace tina;
 
-namespace {
-
-constexpr int kBlock = 256;
-constexpr int kSmallArea = 64;      // bbox pixels scanned by the face's own lane
-constexpr int kGiantArea = 16384;   // bbox pixels above which all workgroups share a face
-constexpr int kBigGrid = 2048;      // persistent grid of the big / giant passes
-constexpr int kShadePix = 1024;     // pixels per shade workgroup
-
-struct BigRec {
-  FaceSetup s;     // 72 B
-  uint32_t id;
-  uint32_t pad;
-};
-static_assert(sizeof(BigRec) == 80, "BigRec layout");
-
-struct Counters {
-  uint32_t big, giant, done_big, done_giant;
-};
-
-__global__ void __launch_bounds__(kBlock) k_clear(uint64_t* __restrict__ keys, long n) {
-  const uint64_t e = empty_key();
-  const long stride = (long)gridDim.x * kBlock;
-  const long n2 = n >> 1;
-  ulonglong2* k2 = reinterpret_cast<ulonglong2*>(keys);
-  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n2; i += stride) k2[i] = make_ulonglong2(e, e);
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) keys[n - 1] = e;
-}
-
-__device__ __forceinline__ void cover_pixel(const FaceSetup& s, const Cam& cam, int px, int py, uint32_t id,
-                                            uint64_t* __restrict__ keys) {
-  float w0, w1, w2;
-  if (pixel_weights(s, (float)px + cam.bias[0], (float)py + cam.bias[1], w0, w1, w2)) {
-    const uint64_t k = pack_key(pixel_depth(s, w0, w1, w2), id);
-    atomicMin((unsigned long long*)(keys + (size_t)py * cam.W + px), (unsigned long long)k);
-  }
-}
-
-__device__ __forceinline__ void push_rec(bool take, const FaceSetup& s, uint32_t id, BigRec* __restrict__ q,
-                                         uint32_t* __restrict__ count) {
-  const unsigned long long mask = __ballot(take);
-  if (mask == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  if (take) {
-    const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    BigRec r;
-    r.s = s;
-    r.id = id;
-    r.pad = 0;
-    q[idx] = r;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock)
-k_raster_small(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, Cam cam,
-               uint64_t* __restrict__ keys, BigRec* __restrict__ bigq, BigRec* __restrict__ giantq,
-               Counters* __restrict__ ctr) {
-  __shared__ float sv[kBlock * 9];
-  const int tid = threadIdx.x;
-  const long f0 = (long)blockIdx.x * kBlock;
-  const int nf = (int)min((long)kBlock, (long)T - f0);
-  // stage this block's faces through LDS with coalesced loads (36 B per face)
-  const float* src = verts + f0 * 9;
-  for (int i = tid; i < nf * 9; i += kBlock) sv[i] = src[i];
-  __syncthreads();
-
-  bool big = false, giant = false;
-  FaceSetup s;
-  const uint32_t id = id_base + (uint32_t)(f0 + tid);
-  if (tid < nf && setup_face(cam, sv + tid * 9, flags, s)) {
-    const int w = s.topx - s.botx + 1, h = s.topy - s.boty + 1;
-    if (w > 0 && h > 0) {
-      const long area = (long)w * h;
-      if (area <= kSmallArea) {
-        for (int py = s.boty; py <= s.topy; py++)
-          for (int px = s.botx; px <= s.topx; px++) cover_pixel(s, cam, px, py, id, keys);
-      } else if (area <= kGiantArea) {
-        big = true;
-      } else {
-        giant = true;
-      }
-    }
-  }
-  push_rec(big, s, id, bigq, &ctr->big);
-  push_rec(giant, s, id, giantq, &ctr->giant);
-}
-
-// One workgroup per big face (persistent grid); lanes stride over the bbox.
-__global__ void __launch_bounds__(kBlock)
-k_raster_big(const BigRec* __restrict__ q, Counters* __restrict__ ctr, Cam cam, uint64_t* __restrict__ keys) {
-  __shared__ uint32_t s_count;
-  if (threadIdx.x == 0) s_count = __atomic_load_n(&ctr->big, __ATOMIC_RELAXED);
-  __syncthreads();
-  const uint32_t count = s_count;
-  for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
-    const BigRec rec = q[r];
-    const int w = rec.s.topx - rec.s.botx + 1;
-    const int area = w * (rec.s.topy - rec.s.boty + 1);
-    for (int i = threadIdx.x; i < area; i += kBlock) {
-      const int dy = i / w;
-      cover_pixel(rec.s, cam, rec.s.botx + (i - dy * w), rec.s.boty + dy, rec.id, keys);
-    }
-  }
-  // the last workgroup to finish resets the queue for the next raster call;
-  // every workgroup has read the count before it takes a ticket
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t t = atomicAdd(&ctr->done_big, 1u);
-    if (t == gridDim.x - 1) {
-      __atomic_store_n(&ctr->big, 0u, __ATOMIC_RELAXED);
-      __atomic_store_n(&ctr->done_big, 0u, __ATOMIC_RELAXED);
-    }
-  }
-}
-
-// Giant faces: every workgroup takes a slice of every giant face.
-__global__ void __launch_bounds__(kBlock)
-k_raster_giant(const BigRec* __restrict__ q, Counters* __restrict__ ctr, Cam cam, uint64_t* __restrict__ keys) {
-  __shared__ uint32_t s_count;
-  if (threadIdx.x == 0) s_count = __atomic_load_n(&ctr->giant, __ATOMIC_RELAXED);
-  __syncthreads();
-  const uint32_t count = s_count;
-  const long stride = (long)gridDim.x * kBlock;
-  for (uint32_t r = 0; r < count; r++) {
-    const BigRec rec = q[r];
-    const long w = rec.s.topx - rec.s.botx + 1;
-    const long area = w * (long)(rec.s.topy - rec.s.boty + 1);
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < area; i += stride) {
-      const long dy = i / w;
-      cover_pixel(rec.s, cam, rec.s.botx + (int)(i - dy * w), rec.s.boty + (int)dy, rec.id, keys);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t t = atomicAdd(&ctr->done_giant, 1u);
-    if (t == gridDim.x - 1) {
-      __atomic_store_n(&ctr->giant, 0u, __ATOMIC_RELAXED);
-      __atomic_store_n(&ctr->done_giant, 0u, __ATOMIC_RELAXED);
-    }
-  }
-}
-
-__device__ __forceinline__ void store_rgb(float* __restrict__ img, long P, V3 c) {
-  img[3 * P] = c.x;
-  img[3 * P + 1] = c.y;
-  img[3 * P + 2] = c.z;
-}
-
-__global__ void __launch_bounds__(kBlock)
-k_shade(const ShadeArgs a, const uint64_t* __restrict__ keys, float* __restrict__ img) {
-  __shared__ uint32_t s_pix[kShadePix];
-  __shared__ uint32_t s_face[kShadePix];
-  __shared__ uint32_t s_n;
-  const int tid = threadIdx.x;
-  const long npix = (long)a.cam.W * a.cam.H;
-  const long base = (long)blockIdx.x * kShadePix;
-  if (tid == 0) s_n = 0;
-  __syncthreads();
-  const int lane = __lane_id();
-  const unsigned long long lt_mask = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int r = 0; r < kShadePix / kBlock; r++) {
-    const long P = base + r * kBlock + tid;
-    bool take = false;
-    uint32_t f = 0;
-    if (P < npix) {
-      const uint64_t k = keys[P];
-      if (key_depth(k) >= TINA_MAXDEPTH) {
-        if (a.flags & TINA_BG) store_rgb(img, P, background(a, (int)(P % a.cam.W), (int)(P / a.cam.W)));
-      } else {
-        f = (uint32_t)k - a.mesh.id_base;
-        take = f < (uint32_t)a.mesh.nfaces;
-      }
-    }
-    // wave-ballot compaction of the covered pixels into the LDS list
-    const unsigned long long mask = __ballot(take);
-    uint32_t off = 0;
-    if (lane == 0 && mask) off = atomicAdd(&s_n, (uint32_t)__popcll(mask));
-    off = __shfl(off, 0);
-    if (take) {
-      const uint32_t j = off + (uint32_t)__popcll(mask & lt_mask);
-      s_pix[j] = (uint32_t)(P - base);
-      s_face[j] = f;
-    }
-  }
-  __syncthreads();
-  const uint32_t n = s_n;
-  for (uint32_t j = tid; j < n; j += kBlock) {
-    const long P = base + s_pix[j];
-    const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
-    store_rgb(img, P, shade_pixel(a, s_face[j], px, py));
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_fill(const ShadeArgs a, float* __restrict__ img) {
-  const long npix = (long)a.cam.W * a.cam.H;
-  for (long P = (long)blockIdx.x * kBlock + threadIdx.x; P < npix; P += (long)gridDim.x * kBlock)
-    store_rgb(img, P, background(a, (int)(P % a.cam.W), (int)(P / a.cam.W)));
-}
-
-__global__ void __launch_bounds__(kBlock) k_aces(float* __restrict__ img, long n) {
-  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
-    img[i] = aces1(img[i]);
-}
-
-__global__ void __launch_bounds__(kBlock)
-k_resolve(const uint64_t* __restrict__ keys, int32_t* __restrict__ id, int32_t* __restrict__ depth, long n) {
-  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-    const uint64_t k = keys[i];
-    const int32_t d = key_depth(k);
-    if (depth) depth[i] = d;
-    if (id) id[i] = d >= TINA_MAXDEPTH ? -1 : (int32_t)(uint32_t)k;
-  }
-}
-
-}  // namespace
-
-// ============================================================== C ABI
-
 struct tina_ctx {
   int W = 0, H = 0, cap = 0;
+  int tiles_x = 0, tiles_y = 0, groups_cap = 0;
   uint64_t* keys = nullptr;
   BigRec* bigq = nullptr;
   BigRec* giantq = nullptr;
   Counters* ctr = nullptr;
+  // binned path: per (tile, binning group) chunk records, the bin entries, counters
+  uint32_t* chunk_off = nullptr;
+  uint32_t* chunk_cnt = nullptr;
+  uint32_t* entries = nullptr;
+  BinCounters* binctr = nullptr;
   tina_camera cam{};
   std::string err;
 };
@@ -295,7 +71,7 @@ Cam make_cam(const tina_ctx* c) {
   return k;
 }
 
-unsigned grid_for(long n, long per_block, unsigned cap = 65535u * 8u) {
+unsigned grid_for(long n, long per_block, unsigned cap) {
   long g = (n + per_block - 1) / per_block;
   if (g < 1) g = 1;
   if (g > (long)cap) g = cap;
@@ -303,18 +79,37 @@ unsigned grid_for(long n, long per_block, unsigned cap = 65535u * 8u) {
 }
 
 void free_scratch(tina_ctx* c) {
-  if (c->bigq) (void)hipFree(c->bigq);
-  if (c->giantq) (void)hipFree(c->giantq);
+  for (void* p : {(void*)c->bigq, (void*)c->giantq, (void*)c->chunk_off, (void*)c->chunk_cnt, (void*)c->entries})
+    if (p) (void)hipFree(p);
   c->bigq = c->giantq = nullptr;
+  c->chunk_off = c->chunk_cnt = c->entries = nullptr;
+  c->groups_cap = 0;
+}
+
+// The binned path needs its per-tile counters and the group's face spans in LDS.
+bool binned_fits(const tina_ctx* c, int nfaces) {
+  const long ntiles = (long)c->tiles_x * c->tiles_y;
+  const long groups = (nfaces + kBinFaces - 1) / kBinFaces;
+  return (ntiles + kBinFaces) * 4 <= 65536 && groups <= c->groups_cap && groups <= kMaxBinGroups;
 }
 
 int alloc_scratch(tina_ctx* c, int cap) {
   free_scratch(c);
-  size_t bytes = sizeof(BigRec) * (size_t)(cap > 0 ? cap : 1);
-  if (hipMalloc(&c->bigq, bytes) != hipSuccess || hipMalloc(&c->giantq, bytes) != hipSuccess) {
+  const size_t n = (size_t)(cap > 0 ? cap : 1);
+  const size_t ntiles = (size_t)c->tiles_x * c->tiles_y;
+  const size_t groups = (n + kBinFaces - 1) / kBinFaces;
+  bool ok = hipMalloc(&c->bigq, sizeof(BigRec) * n) == hipSuccess &&
+            hipMalloc(&c->giantq, sizeof(BigRec) * n) == hipSuccess;
+  if (ok && (ntiles + kBinFaces) * 4 <= 65536 && groups <= (size_t)kMaxBinGroups) {
+    ok = hipMalloc(&c->chunk_off, 4 * ntiles * groups) == hipSuccess &&
+         hipMalloc(&c->chunk_cnt, 4 * ntiles * groups) == hipSuccess &&
+         hipMalloc(&c->entries, 4 * n * kBinMaxTiles) == hipSuccess;
+    if (ok) c->groups_cap = (int)groups;
+  }
+  if (!ok) {
     free_scratch(c);
     (void)hipGetLastError();
-    return fail(c, TINA_ERR_ALLOC, "hipMalloc of raster queues failed");
+    return fail(c, TINA_ERR_ALLOC, "hipMalloc of the raster bins and queues failed");
   }
   c->cap = cap;
   return TINA_OK;
@@ -344,11 +139,43 @@ int check_material(tina_ctx* c, const tina_material* m) {
     if (p.kind < TINA_P_CONST || p.kind > TINA_P_FRESNEL) return fail(c, TINA_ERR_INVALID, "unknown parameter kind");
     if ((p.kind == TINA_P_PARAM || p.kind == TINA_P_FACE || p.kind == TINA_P_TEXTURE) && !p.ptr)
       return fail(c, TINA_ERR_INVALID, "parameter pointer is NULL");
-    if (p.kind == TINA_P_FRESNEL && !(p.a >= 0 && p.a < i && p.b >= 0 && p.b < i && p.c >= 0 && p.c < i))
-      return fail(c, TINA_ERR_INVALID, "FresnelFactor operands must precede it");
-    if (p.kind == TINA_P_TEXTURE && !(p.a >= 0 && p.a < i && p.tex_w > 0 && p.tex_h > 0 && p.tex_c > 0))
-      return fail(c, TINA_ERR_INVALID, "texture parameter malformed");
+    if (p.kind == TINA_P_FRESNEL) {
+      if (!(p.a >= 0 && p.a < i && p.b >= 0 && p.b < i && p.c >= 0 && p.c < i))
+        return fail(c, TINA_ERR_INVALID, "FresnelFactor operands must precede it");
+      for (int q : {p.a, p.b, p.c})
+        if (m->params[q].kind == TINA_P_FRESNEL)
+          return fail(c, TINA_ERR_INVALID, "FresnelFactor operands must be plain parameters");
+    }
+    if (p.kind == TINA_P_TEXTURE) {
+      if (!(p.a >= 0 && p.a < i && p.tex_w > 0 && p.tex_h > 0 && p.tex_c > 0))
+        return fail(c, TINA_ERR_INVALID, "texture parameter malformed");
+      const int k = m->params[p.a].kind;
+      if (k == TINA_P_TEXTURE || k == TINA_P_FRESNEL)
+        return fail(c, TINA_ERR_INVALID, "texture coordinates must be a plain parameter");
+    }
   }
+  return TINA_OK;
+}
+
+int make_shade_args(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,
+                    const tina_ibl* ibl, const float bg[3], uint32_t flags, ShadeArgs& a) {
+  memset(&a, 0, sizeof(a));
+  if (mesh) a.mesh = *mesh;
+  if (mat) a.mat = *mat;
+  if (lights) {
+    if (lights->n < 0 || lights->n > TINA_MAX_LIGHTS) return fail(c, TINA_ERR_CAPACITY, "light count out of range");
+    a.lights = *lights;
+  }
+  if (ibl) a.ibl = *ibl;
+  if ((flags & TINA_IBL) && (!ibl || !ibl->spec[0] || !ibl->diff))
+    return fail(c, TINA_ERR_INVALID, "TINA_IBL needs the IBL tables");
+  a.cam = make_cam(c);
+  if (bg) {
+    a.bg[0] = bg[0];
+    a.bg[1] = bg[1];
+    a.bg[2] = bg[2];
+  }
+  a.flags = flags;
   return TINA_OK;
 }
 
@@ -370,29 +197,33 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
   if (!c) return fail(nullptr, TINA_ERR_ALLOC, "host allocation failed");
   c->W = width;
   c->H = height;
+  c->tiles_x = (width + kTile - 1) / kTile;
+  c->tiles_y = (height + kTile - 1) / kTile;
   // default camera: W2V = V2W = diag(1, 1, -1, 1), bias 0.5 (engine.py:20-26)
   for (int i = 0; i < 16; i++) c->cam.W2V[i] = c->cam.V2W[i] = (i % 5 == 0) ? 1.0f : 0.0f;
   c->cam.W2V[10] = c->cam.V2W[10] = -1.0f;
   c->cam.bias[0] = c->cam.bias[1] = 0.5f;
-  if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
-      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess) {
-    (void)hipGetLastError();
+  auto release = [c]() {
     if (c->keys) (void)hipFree(c->keys);
+    if (c->ctr) (void)hipFree(c->ctr);
+    if (c->binctr) (void)hipFree(c->binctr);
     delete c;
+  };
+  if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
+      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, sizeof(BinCounters)) != hipSuccess) {
+    (void)hipGetLastError();
+    release();
     return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the depth|id buffer failed");
   }
-  if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-    (void)hipFree(c->keys);
-    (void)hipFree(c->ctr);
-    delete c;
+  if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess ||
+      hipMemset(c->binctr, 0, sizeof(BinCounters)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    release();
     return fail(nullptr, TINA_ERR_HIP, "counter initialisation failed");
   }
   int rc = alloc_scratch(c, max_faces);
   if (rc != TINA_OK) {
     g_create_err = c->err;
-    (void)hipFree(c->keys);
-    (void)hipFree(c->ctr);
-    delete c;
+    release();
     return rc;
   }
   *out = c;
@@ -404,6 +235,7 @@ int tina_ctx_destroy(tina_ctx* c) {
   free_scratch(c);
   if (c->keys) (void)hipFree(c->keys);
   if (c->ctr) (void)hipFree(c->ctr);
+  if (c->binctr) (void)hipFree(c->binctr);
   delete c;
   return TINA_OK;
 }
@@ -431,7 +263,8 @@ int tina_keys(tina_ctx* c, uint64_t** d_keys) {
 int tina_clear(tina_ctx* c, void* stream) {
   if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
   const long n = (long)c->W * c->H;
-  hipLaunchKernelGGL(k_clear, dim3(grid_for(n / 2 + 1, kBlock, 4096)), dim3(kBlock), 0, (hipStream_t)stream, c->keys, n);
+  hipLaunchKernelGGL(k_clear, dim3(grid_for(n / 2 + 1, kBlock, 4096)), dim3(kBlock), 0, (hipStream_t)stream,
+                     c->keys, n);
   TINA_LAUNCH(c, "k_clear");
   return TINA_OK;
 }
@@ -448,32 +281,27 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     return fail(c, TINA_ERR_INVALID, "face ids overflow 32 bits");
   hipStream_t s = (hipStream_t)stream;
   const Cam cam = make_cam(c);
+  if (binned_fits(c, nfaces)) {
+    const int ntiles = c->tiles_x * c->tiles_y;
+    const int ngroups = (nfaces + kBinFaces - 1) / kBinFaces;
+    hipLaunchKernelGGL(k_bin, dim3(ngroups), dim3(kBinBlock), (size_t)(ntiles + kBinFaces) * 4, s, d_verts, nfaces,
+                       id_base, flags, cam, c->tiles_x, ntiles, ngroups, c->chunk_off, c->chunk_cnt, c->entries,
+                       c->giantq, c->binctr);
+    TINA_LAUNCH(c, "k_bin");
+    hipLaunchKernelGGL(k_giant, dim3(kQueueGrid), dim3(kBlock), 0, s, c->giantq, c->binctr, cam, c->keys);
+    TINA_LAUNCH(c, "k_giant");
+    hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(kBlock), (size_t)(ngroups + 1) * 4, s, d_verts, id_base, flags,
+                       cam, c->tiles_x, ngroups, c->chunk_off, c->chunk_cnt, c->entries, c->keys, c->binctr);
+    TINA_LAUNCH(c, "k_tile");
+    return TINA_OK;
+  }
+  // resolutions or face counts beyond the binned path's LDS tables: global atomics
   hipLaunchKernelGGL(k_raster_small, dim3(grid_for(nfaces, kBlock, 0x7fffffff)), dim3(kBlock), 0, s, d_verts,
                      nfaces, id_base, flags, cam, c->keys, c->bigq, c->giantq, c->ctr);
   TINA_LAUNCH(c, "k_raster_small");
-  hipLaunchKernelGGL(k_raster_big, dim3(kBigGrid), dim3(kBlock), 0, s, c->bigq, c->ctr, cam, c->keys);
-  TINA_LAUNCH(c, "k_raster_big");
-  hipLaunchKernelGGL(k_raster_giant, dim3(kBigGrid), dim3(kBlock), 0, s, c->giantq, c->ctr, cam, c->keys);
-  TINA_LAUNCH(c, "k_raster_giant");
-  return TINA_OK;
-}
-
-static int make_shade_args(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,
-                           const tina_ibl* ibl, const float bg[3], uint32_t flags, ShadeArgs& a) {
-  memset(&a, 0, sizeof(a));
-  if (mesh) a.mesh = *mesh;
-  if (mat) a.mat = *mat;
-  if (lights) {
-    if (lights->n < 0 || lights->n > TINA_MAX_LIGHTS) return fail(c, TINA_ERR_CAPACITY, "light count out of range");
-    a.lights = *lights;
-  }
-  if (ibl) a.ibl = *ibl;
-  if ((flags & TINA_IBL)) {
-    if (!ibl || !ibl->spec[0]) return fail(c, TINA_ERR_INVALID, "TINA_IBL needs the IBL tables");
-  }
-  a.cam = make_cam(c);
-  if (bg) { a.bg[0] = bg[0]; a.bg[1] = bg[1]; a.bg[2] = bg[2]; }
-  a.flags = flags;
+  hipLaunchKernelGGL(k_raster_queue, dim3(kQueueGrid), dim3(kBlock), 0, s, c->bigq, c->giantq, c->ctr, cam,
+                     c->keys);
+  TINA_LAUNCH(c, "k_raster_queue");
   return TINA_OK;
 }
 
@@ -492,8 +320,9 @@ int tina_shade(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, con
   rc = make_shade_args(c, mesh, mat, lights, ibl, bg, flags, a);
   if (rc) return rc;
   const long npix = (long)c->W * c->H;
-  hipLaunchKernelGGL(k_shade, dim3(grid_for(npix, kShadePix, 0x7fffffff)), dim3(kBlock), 0, (hipStream_t)stream, a,
-                     c->keys, d_img);
+  const bool ibl_mode = (flags & TINA_IBL) != 0;
+  dispatch_shade(classify_program(a.mat, ibl_mode), ibl_mode, grid_for(npix, kShadePix, 0x7fffffff),
+                 (hipStream_t)stream, a, c->keys, d_img);
   TINA_LAUNCH(c, "k_shade");
   return TINA_OK;
 }
@@ -522,8 +351,8 @@ int tina_resolve(tina_ctx* c, int32_t* d_id, int32_t* d_depth, void* stream) {
   if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
   if (!d_id && !d_depth) return TINA_OK;
   const long n = (long)c->W * c->H;
-  hipLaunchKernelGGL(k_resolve, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, c->keys, d_id,
-                     d_depth, n);
+  hipLaunchKernelGGL(k_resolve, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, c->keys,
+                     d_id, d_depth, n);
   TINA_LAUNCH(c, "k_resolve");
   return TINA_OK;
 }

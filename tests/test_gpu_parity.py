@@ -108,6 +108,46 @@ def test_raster_sizes_and_culling(gpu, edge_mu, seed):
         assert np.array_equal(depth.cpu().numpy(), od)
 
 
+def test_raster_thin_and_screen_filling_faces(gpu):
+    """Bboxes one pixel wide or tall (the queue pass's row/column split), faces larger
+    than the screen with a vertex inside the NDC cube, and faces crossing w = 0
+    (no near clipping in the reference: culled or clipped as it decides)."""
+    import torch
+    import taichi_three_amd as tina
+    W, H = 160, 1200
+    M = np.diag([1.0, 1.0, -1.0, 1.0])
+    xs = np.linspace(-0.9, 0.9, 40)
+    faces = []
+    for i, x in enumerate(xs):
+        dx = (i % 3) * 0.004 + 0.0005                      # 0.04 to 0.7 px wide
+        faces.append([[x, -0.95, 0.1 * (i % 5)], [x + dx, -0.95, 0.0], [x, 0.95, 0.2]])
+        faces.append([[-0.95, x, 0.3], [0.95, x, 0.1], [-0.95, x + dx / 8, -0.1]])
+    faces.append([[-3, -3, 0.5], [3, -3, 0.5], [0, 0.5, 0.5]])   # larger than the screen
+    faces.append([[-0.5, -0.5, 0.2], [0.5, -0.5, 0.2], [0.0, 0.5, -1.5]])  # one vertex beyond z = -1
+    verts = np.array(faces, dtype=np.float32)
+    rng = np.random.default_rng(3)
+    persp = np.array([[1.0, 0, 0, 0], [0, 1.0, 0, 0], [0, 0, -1.0, -0.1], [0, 0, -1.0, 0.5]])
+    for name, W2V in (('identity', M), ('projective', persp)):
+        for culling in (True, False):
+            print(f'case: thin faces {name} culling={culling}', flush=True)
+            eng = tina.Engine(W, H, maxfaces=len(verts))
+            eng.W2V = W2V.astype(np.float32)
+            eng.V2W = np.linalg.inv(W2V).astype(np.float32)
+            eng._push_camera()
+            r = tina.TriangleRaster(eng, maxfaces=len(verts), culling=culling)
+            m = tina.SimpleMesh()
+            m.set_face_verts(verts[rng.permutation(len(verts))] if name == 'projective' else verts)
+            eng.clear_depth()
+            r.set_object(m)
+            r.render_occup()
+            ids, depth = eng.resolve()
+            torch.cuda.synchronize()
+            cam = orc.camera(eng.W2V, eng.V2W)
+            od, oi = orc.raster(W, H, cam, m.verts.cpu().numpy(), flags=(1 if culling else 0) | 2)
+            assert np.array_equal(ids.cpu().numpy(), oi)
+            assert np.array_equal(depth.cpu().numpy(), od)
+
+
 def test_multi_object_ties_and_probe(gpu):
     """Two objects with coincident faces: the earlier object wins every tie; the
     probe reports local face indices of the winners."""
@@ -153,6 +193,54 @@ def test_deterministic_repeat(gpu):
     for i in range(1, 3):
         assert np.array_equal(out[0][0], out[i][0])
         assert np.array_equal(out[0][1], out[i][1])
+
+
+def test_generic_material_program_and_nodes(gpu):
+    """A material tree no fixed kernel matches (the interpreter path), with Param,
+    Texture (texturing=True), Input('normal') and an Emission lobe."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import workloads as wl, _abi
+    rng = np.random.default_rng(21)
+    W, H = 200, 150
+    verts = wl.triangle_soup(rng, 4000, np.log(0.15), 0.5)
+    coors = rng.uniform(0, 1, size=(len(verts), 3, 2)).astype(np.float32)
+    tex = rng.uniform(0, 1, size=(16, 8, 3)).astype(np.float32)
+    rough = tina.Param(initial=0.3)
+    mat = tina.MixMaterial(tina.Lambert() * tina.Texture(tex),
+                           tina.CookTorrance(roughness=rough, fresnel=0.5) + tina.Phong(shineness=8) * 'normal',
+                           0.5) + tina.Emission() * 0.05
+    scene = tina.Scene(W, H, texturing=True, maxfaces=len(verts))
+    view, proj = wl.default_camera(W / H)
+    scene.engine.set_camera(view, proj)
+    mesh = tina.SimpleMesh()
+    mesh.set_face_verts(verts)
+    mesh.set_face_coors(coors)
+    scene.add_object(mesh, mat)
+    cam = orc.camera(scene.engine.W2V, scene.engine.V2W)
+    for r in (0.3, 0.7):
+        rough.value[None] = r
+        print(f'case: generic material roughness {r}', flush=True)
+        scene.render()
+        torch.cuda.synchronize()
+        ids = scene.engine.ids.to_numpy().T
+        img = scene.img.to_numpy().transpose(1, 0, 2)
+        _, oi = orc.raster(W, H, cam, verts)
+        assert np.array_equal(ids, oi)
+        # the oracle reads Param / Texture from host copies of the same values
+        lo = tina.matr.lower_material(mat, dict(attrs={}))
+        host = []
+        for i in range(lo.struct.nparams):
+            p = lo.struct.params[i]
+            if p.kind in (_abi.P_PARAM, _abi.P_TEXTURE):
+                src = rough.tensor if p.kind == _abi.P_PARAM else mat.mat1.mat1.params['factor'].tensor
+                h = np.ascontiguousarray(src.cpu().numpy())
+                host.append(h)
+                p.ptr = h.ctypes.data
+        flags = _abi.TEX | _abi.TONEMAP | _abi.BG
+        o32 = orc.shade(W, H, cam, orc.HostMesh(verts, None, coors, None), lo.struct,
+                        scene.lighting.to_struct(), oi, flags)
+        assert np.abs(img - o32).max() <= COLOR_TOL_F32
 
 
 def test_plain_512_matches_golden_fixture(gpu):
