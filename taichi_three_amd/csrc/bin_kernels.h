@@ -2,20 +2,22 @@
 // (tina/core/triangle.py:88-131).
 //
 //   k_bin    one 1024-lane workgroup per kBinFaces consecutive faces.  Each face is
-//            set up (cull, clip, bbox) and counted into every kTile x kTile screen
-//            tile its bbox touches, in an LDS histogram.  One global atomic reserves
-//            the workgroup's block of bin entries; the workgroup writes, per tile, its
-//            (offset, count) chunk record and scatters its face ids into its block.
-//            Faces whose bbox touches more than kBinMaxTiles tiles go to the giant
-//            queue (k_giant, global atomics) instead.
+//            set up once (cull, clip, bbox, edge coefficients; triangle.py:93-113)
+//            and its setup record stored; it is counted into every kTile x kTile
+//            screen tile its bbox touches, in an LDS histogram.  One global atomic
+//            reserves the workgroup's block of bin entries; the workgroup writes,
+//            per tile, its (offset, count) chunk record and scatters its face ids
+//            into its block.  Faces whose bbox touches more than kBinMaxTiles
+//            tiles go to the giant queue (k_giant, global atomics) instead.
 //   k_tile   one workgroup per screen tile.  The tile's chunks from all binning
 //            workgroups form one virtual list (prefix sum of their counts in LDS).
 //            The tile's keys are loaded into LDS once; each wave then pulls 64 faces
-//            at a time from the list, sets them up, clips their bboxes to the tile,
-//            and rasterises them with the wave-cooperative fragment expansion,
-//            reducing every covered pixel with an LDS 64-bit atomicMin.  Waves run
-//            independently (no workgroup barrier inside the loop).  The keys go
-//            back to HBM once, coalesced.
+//            at a time from the list, reads their setup records, clips their bboxes
+//            to the tile and rasterises them with the wave-cooperative fragment
+//            expansion (the bbox pixels of the wave's faces laid end to end, one
+//            pixel per lane per step), reducing every covered pixel with an LDS
+//            64-bit atomicMin.  Waves run independently (no workgroup barrier in
+//            the loop).  The keys go back to HBM once, coalesced.
 //
 // No global atomic touches the key buffer on this path: the 64-bit atomic rate
 // to HBM-resident lines (about 27 G/s measured on MI355X) would bound the pass.
@@ -24,15 +26,24 @@
 
 namespace tina {
 
-constexpr int kTileLog = 5;
-constexpr int kTile = 1 << kTileLog;  // 32 x 32 pixel tiles, 8 KB of keys in LDS
+constexpr int kTileLog = 4;
+constexpr int kTile = 1 << kTileLog;  // 16 x 16 pixel tiles, 2 KB of keys in LDS
 constexpr int kBinBlock = 1024;       // lanes of a binning workgroup
 constexpr int kBinFaces = 4096;       // faces per binning workgroup (4 per lane)
-constexpr int kBinMaxTiles = 16;      // bbox tiles above which a face is rasterised as a giant
+constexpr int kBinMaxTiles = 64;      // bbox tiles above which a face is rasterised as a giant
 constexpr int kMaxBinGroups = 8192;   // binning workgroups a k_tile workgroup can index
 
 struct BinCounters {
   uint32_t cursor, giant, pad0, pad1;
+};
+
+// Setup record of a binned face (64 B, one cache line): the state the coverage
+// loop needs (triangle.py:127-131 plus the view-space z of A, B, C) and its bbox.
+struct FaceRec {
+  float4 a;  // bx, by, cx, cy
+  float4 b;  // bcnx, bcny, canx, cany
+  float4 c;  // ws0, ws1, ws2, Az
+  float4 d;  // Bz, Cz, botx | boty << 16, topx | topy << 16
 };
 
 // The tiles a face's bbox touches: x0 (12 bits), y0 (12 bits), x span - 1 (4 bits),
@@ -41,8 +52,8 @@ __device__ __forceinline__ uint32_t pack_span(int tx0, int ty0, int nx, int ny) 
   return (uint32_t)tx0 | ((uint32_t)ty0 << 12) | ((uint32_t)(nx - 1) << 24) | ((uint32_t)(ny - 1) << 28);
 }
 
-// Block-wide exclusive scan of n values v(i) into out(i), nthreads lanes at a time;
-// returns the total.  wsum: LDS scratch of nthreads/64 words; carry: one LDS word.
+// Block-wide exclusive scan of n values v(i) into out(i), NT lanes at a time;
+// returns the total.  wsum: LDS scratch of NT/64 words; carry: one LDS word.
 template <int NT, class Get, class Put>
 __device__ __forceinline__ uint32_t block_exclusive_scan(int n, Get v, Put out, uint32_t* wsum, uint32_t* carry) {
   const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
@@ -70,7 +81,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(int n, Get v, Put out, 
 __global__ void __launch_bounds__(kBinBlock)
 k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, Cam cam, int tiles_x, int ntiles,
       int ngroups, uint32_t* __restrict__ chunk_off, uint32_t* __restrict__ chunk_cnt, uint32_t* __restrict__ entries,
-      BigRec* __restrict__ giantq, BinCounters* __restrict__ ctr) {
+      FaceRec* __restrict__ recs, BigRec* __restrict__ giantq, BinCounters* __restrict__ ctr) {
   extern __shared__ uint32_t lds[];
   uint32_t* cnt = lds;            // [ntiles] counts, then running offsets
   uint32_t* span = lds + ntiles;  // [kBinFaces] per-face tile span
@@ -90,10 +101,17 @@ k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, 
       const int tx0 = s.botx >> kTileLog, tx1 = s.topx >> kTileLog;
       const int ty0 = s.boty >> kTileLog, ty1 = s.topy >> kTileLog;
       const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1;
-      if (nx * ny > kBinMaxTiles) {
+      if (nx * ny > kBinMaxTiles || nx > 16 || ny > 16) {
         giant = true;
       } else {
         sp = pack_span(tx0, ty0, nx, ny);
+        FaceRec r;
+        r.a = make_float4(s.bx, s.by, s.cx, s.cy);
+        r.b = make_float4(s.bcnx, s.bcny, s.canx, s.cany);
+        r.c = make_float4(s.ws0, s.ws1, s.ws2, s.Az);
+        r.d = make_float4(s.Bz, s.Cz, __int_as_float(s.botx | (s.boty << 16)),
+                          __int_as_float(s.topx | (s.topy << 16)));
+        recs[f] = r;
         for (int y = ty0; y <= ty1; y++)
           for (int x = tx0; x <= tx1; x++) atomicAdd(&cnt[y * tiles_x + x], 1u);
       }
@@ -106,14 +124,14 @@ k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, 
   const uint32_t total = block_exclusive_scan<kBinBlock>(
       ntiles, [&](int t) { return cnt[t]; },
       [&](int t, uint32_t ofs) {
-        chunk_cnt[(size_t)t * ngroups + g] = cnt[t];
+        chunk_cnt[(size_t)g * ntiles + t] = cnt[t];
         cnt[t] = ofs;
       },
       s_wsum, &s_carry);
   if (tid == 0) s_base = atomicAdd(&ctr->cursor, total);
   __syncthreads();
   const uint32_t base = s_base;
-  for (int t = tid; t < ntiles; t += kBinBlock) chunk_off[(size_t)t * ngroups + g] = base + cnt[t];
+  for (int t = tid; t < ntiles; t += kBinBlock) chunk_off[(size_t)g * ntiles + t] = base + cnt[t];
   __syncthreads();
 
   // scatter the face ids (order within a tile is irrelevant: the reduction is a min)
@@ -155,12 +173,12 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_tile(const float* __restrict__ verts, uint32_t id_base, uint32_t flags, Cam cam, int tiles_x, int ngroups,
-       const uint32_t* __restrict__ chunk_off, const uint32_t* __restrict__ chunk_cnt,
-       const uint32_t* __restrict__ entries, uint64_t* __restrict__ keys, BinCounters* __restrict__ ctr) {
+k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __restrict__ chunk_off,
+       const uint32_t* __restrict__ chunk_cnt, const uint32_t* __restrict__ entries,
+       const FaceRec* __restrict__ recs, uint64_t* __restrict__ keys, BinCounters* __restrict__ ctr) {
   extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
   __shared__ unsigned long long tk[kTile * kTile];
-  __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock], R4[kBlock];
+  __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
   __shared__ int32_t Lend[kBlock];
   __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
@@ -172,9 +190,10 @@ k_tile(const float* __restrict__ verts, uint32_t id_base, uint32_t flags, Cam ca
     ctr->cursor = 0;
     ctr->giant = 0;
   }
-  const uint32_t* cc = chunk_cnt + (size_t)t * ngroups;
+  const int ntiles = gridDim.x;  // chunk tables are group-major: [group][tile]
   const uint32_t total = block_exclusive_scan<kBlock>(
-      ngroups, [&](int c) { return cc[c]; }, [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);
+      ngroups, [&](int c) { return chunk_cnt[(size_t)c * ntiles + t]; },
+      [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);
   if (total == 0) return;
   if (tid == 0) {
     pre[ngroups] = total;
@@ -187,7 +206,6 @@ k_tile(const float* __restrict__ verts, uint32_t id_base, uint32_t flags, Cam ca
   }
   __syncthreads();
 
-  const uint32_t* co = chunk_off + (size_t)t * ngroups;
   for (;;) {
     uint32_t b0 = 0;
     if (lane == 0) b0 = atomicAdd(&s_next, 64u);
@@ -201,50 +219,58 @@ k_tile(const float* __restrict__ verts, uint32_t id_base, uint32_t flags, Cam ca
         const int mid = (lo + hi) >> 1;
         if (pre[mid] <= v) lo = mid; else hi = mid;
       }
-      const uint32_t fid = entries[co[lo] + (v - pre[lo])];
-      FaceSetup s;
-      if (setup_face(cam, verts + (size_t)fid * 9, flags, s)) {
-        const int bx0 = max(s.botx, ox), bx1 = min(s.topx, ox + tw - 1);
-        const int by0 = max(s.boty, oy), by1 = min(s.topy, oy + th - 1);
-        const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
-        if (w > 0 && h > 0) {
-          area = w * h;
-          R0[tid] = make_float4(s.bx, s.by, s.cx, s.cy);
-          R1[tid] = make_float4(s.bcnx, s.bcny, s.canx, s.cany);
-          R2[tid] = make_float4(s.ws0, s.ws1, s.ws2, s.Az);
-          R3[tid] = make_float4(s.Bz, s.Cz, __int_as_float(bx0), __int_as_float(by0));
-          R4[tid] = make_float4(__int_as_float(w), 1.0f / (float)w, __int_as_float((int)(id_base + fid)), 0.0f);
-        }
+      const uint32_t fid = entries[chunk_off[(size_t)lo * ntiles + t] + (v - pre[lo])];
+      const FaceRec r = recs[fid];
+      const uint32_t lo16 = (uint32_t)__float_as_int(r.d.z), hi16 = (uint32_t)__float_as_int(r.d.w);
+      const int bx0 = max((int)(lo16 & 0xFFFF), ox), by0 = max((int)(lo16 >> 16), oy);
+      const int bx1 = min((int)(hi16 & 0xFFFF), ox + tw - 1), by1 = min((int)(hi16 >> 16), oy + th - 1);
+      const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
+      if (w > 0 && h > 0) {
+        area = w * h;
+        R0[tid] = r.a;
+        R1[tid] = r.b;
+        R2[tid] = r.c;
+        // Bz, Cz, clipped origin (x | y << 16), width | face id ... packed below
+        R3[tid] = make_float4(r.d.x, r.d.y, __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16)),
+                              __int_as_float((int)(id_base + fid)));
       }
     }
     const int end = wave_inclusive_scan(area, lane);
     Lend[tid] = end;
     const int wtotal = __shfl(end, 63, 64);
     wave_sync_lds();
-    for (int gI = lane; gI < wtotal; gI += 64) {
+    // One fragment: find its face (binary search over the wave's prefix ends), fetch
+    // the face's record, test the pixel, reduce the key in LDS.
+    auto frag = [&](int gI) {
       int k = 0;
 #pragma unroll
       for (int step = 32; step > 0; step >>= 1)
         if (Lend[wbase + k + step - 1] <= gI) k += step;
       const int j = wbase + k;
       const int start = k > 0 ? Lend[j - 1] : 0;
-      const float4 r4 = R4[j];
-      const int wj = __float_as_int(r4.x);
+      const float4 r3 = R3[j];
+      const int packed = __float_as_int(r3.z);
+      const int lx0 = packed & 0xFF, ly0 = (packed >> 8) & 0xFF, wj = packed >> 16;
       const int li = gI - start;
-      const int dy = (int)(((float)li + 0.5f) * r4.y);  // li / wj, exact (li < 1024)
+      const int dy = (int)(((float)li + 0.5f) / (float)wj);  // li / wj, exact (li < 256)
       const int dx = li - dy * wj;
-      const float4 r0 = R0[j], r1 = R1[j], r2 = R2[j], r3 = R3[j];
+      const float4 r0 = R0[j], r1 = R1[j], r2 = R2[j];
       FaceSetup q;
       q.bx = r0.x; q.by = r0.y; q.cx = r0.z; q.cy = r0.w;
       q.bcnx = r1.x; q.bcny = r1.y; q.canx = r1.z; q.cany = r1.w;
       q.ws0 = r2.x; q.ws1 = r2.y; q.ws2 = r2.z; q.Az = r2.w;
       q.Bz = r3.x; q.Cz = r3.y;
-      const int px = __float_as_int(r3.z) + dx, py = __float_as_int(r3.w) + dy;
+      const int lx = lx0 + dx, ly = ly0 + dy;
       float w0, w1, w2;
-      if (pixel_cover_fast(q, (float)px + cam.bias[0], (float)py + cam.bias[1], w0, w1, w2)) {
-        const uint64_t key = pack_key(pixel_depth(q, w0, w1, w2), (uint32_t)__float_as_int(r4.z));
-        atomicMin(&tk[((py - oy) << kTileLog) + (px - ox)], (unsigned long long)key);
+      if (pixel_cover_fast(q, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1], w0, w1, w2)) {
+        const uint64_t key = pack_key(pixel_depth(q, w0, w1, w2), (uint32_t)__float_as_int(r3.w));
+        atomicMin(&tk[(ly << kTileLog) + lx], (unsigned long long)key);
       }
+    };
+    // two independent fragments per lane per step, so their LDS latencies overlap
+    for (int gI = lane; gI < wtotal; gI += 128) {
+      frag(gI);
+      if (gI + 64 < wtotal) frag(gI + 64);
     }
     wave_sync_lds();
   }

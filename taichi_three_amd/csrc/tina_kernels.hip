@@ -28,6 +28,7 @@ struct tina_ctx {
   uint32_t* chunk_off = nullptr;
   uint32_t* chunk_cnt = nullptr;
   uint32_t* entries = nullptr;
+  FaceRec* recs = nullptr;
   BinCounters* binctr = nullptr;
   tina_camera cam{};
   std::string err;
@@ -79,10 +80,11 @@ unsigned grid_for(long n, long per_block, unsigned cap) {
 }
 
 void free_scratch(tina_ctx* c) {
-  for (void* p : {(void*)c->bigq, (void*)c->giantq, (void*)c->chunk_off, (void*)c->chunk_cnt, (void*)c->entries})
+  for (void* p : {(void*)c->bigq, (void*)c->giantq, (void*)c->chunk_off, (void*)c->chunk_cnt, (void*)c->entries, (void*)c->recs})
     if (p) (void)hipFree(p);
   c->bigq = c->giantq = nullptr;
   c->chunk_off = c->chunk_cnt = c->entries = nullptr;
+  c->recs = nullptr;
   c->groups_cap = 0;
 }
 
@@ -103,7 +105,8 @@ int alloc_scratch(tina_ctx* c, int cap) {
   if (ok && (ntiles + kBinFaces) * 4 <= 65536 && groups <= (size_t)kMaxBinGroups) {
     ok = hipMalloc(&c->chunk_off, 4 * ntiles * groups) == hipSuccess &&
          hipMalloc(&c->chunk_cnt, 4 * ntiles * groups) == hipSuccess &&
-         hipMalloc(&c->entries, 4 * n * kBinMaxTiles) == hipSuccess;
+         hipMalloc(&c->entries, 4 * n * kBinMaxTiles) == hipSuccess &&
+         hipMalloc(&c->recs, sizeof(FaceRec) * n) == hipSuccess;
     if (ok) c->groups_cap = (int)groups;
   }
   if (!ok) {
@@ -286,12 +289,12 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     const int ngroups = (nfaces + kBinFaces - 1) / kBinFaces;
     hipLaunchKernelGGL(k_bin, dim3(ngroups), dim3(kBinBlock), (size_t)(ntiles + kBinFaces) * 4, s, d_verts, nfaces,
                        id_base, flags, cam, c->tiles_x, ntiles, ngroups, c->chunk_off, c->chunk_cnt, c->entries,
-                       c->giantq, c->binctr);
+                       c->recs, c->giantq, c->binctr);
     TINA_LAUNCH(c, "k_bin");
     hipLaunchKernelGGL(k_giant, dim3(kQueueGrid), dim3(kBlock), 0, s, c->giantq, c->binctr, cam, c->keys);
     TINA_LAUNCH(c, "k_giant");
-    hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(kBlock), (size_t)(ngroups + 1) * 4, s, d_verts, id_base, flags,
-                       cam, c->tiles_x, ngroups, c->chunk_off, c->chunk_cnt, c->entries, c->keys, c->binctr);
+    hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(kBlock), (size_t)(ngroups + 1) * 4, s, id_base, cam, c->tiles_x,
+                       ngroups, c->chunk_off, c->chunk_cnt, c->entries, c->recs, c->keys, c->binctr);
     TINA_LAUNCH(c, "k_tile");
     return TINA_OK;
   }
