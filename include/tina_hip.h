@@ -199,6 +199,63 @@ int tina_resolve(tina_ctx *ctx, int32_t *d_id, int32_t *d_depth, void *stream);
 /* Device pointer of the packed u64 key buffer ([H][W]) for zero-copy consumers. */
 int tina_keys(tina_ctx *ctx, uint64_t **d_keys);
 
+/* ---- one-time IBL bakes -------------------------------------------------- */
+
+/* One prefiltered equirect level of the IBL tables from the environment map
+ * d_env ([ew][eh][3] f32) into d_out ([ow][oh][3] f32):
+ * Lambert.cook_for_ibl (tina/matr/material.py:407-438) when roughness < 0,
+ * CookTorrance.cook_for_ibl at that roughness (material.py:279-320) otherwise.
+ * nsamples Hammersley points replace ti.random; maps need w, h >= 2.
+ * Errors are reported through tina_last_error(NULL). */
+int tina_bake_ibl(const float *d_env, int ew, int eh, float *d_out, int ow, int oh,
+                  float roughness, int nsamples, void *stream);
+
+/* ---- learned shading surrogate ------------------------------------------- */
+/* No counterpart in the reference: a 12-64-64-64-4 ReLU network trained on the
+ * analytic PBR material (tina/matr/material.py:322-362, 701-706) under
+ * SkyboxLighting plus one directional light (tina/core/lighting.py:4-21, 84-98).
+ * Inputs per sample: n, R = 2 (n.V) n - V, R.L (L = light 0), log(roughness),
+ * metallic, albedo.  Outputs: the ACES-mapped colour and an estimate of its
+ * largest channel error.  Parameters are TINA_NN_PARAMS floats: W1[64][12] b1[64]
+ * W2[64][64] b2[64] W3[64][64] b3[64] W4[4][64] b4[4].  Errors are reported
+ * through tina_last_error(NULL) by the context-free calls. */
+#define TINA_NN_PARAMS 9412
+#define TINA_NN_PACK_BYTES (22 * 64 * 16 + 196 * 4)   /* bf16 MFMA fragments + f32 biases */
+#define TINA_NN_BLOCK 64                               /* samples per gradient row */
+
+/* Training pairs: d_x [n][12] features, d_y [n][4] (ACES colour, 0) from the
+ * analytic shader at hashed inputs of `seed`: n uniform on the sphere, V uniform
+ * on its hemisphere, roughness U[r_lo, r_hi], metallic Bernoulli(p_metal)
+ * (uniform if p_metal < 0), albedo U[0,1]^3, FresnelFactor specular `specular`. */
+int tina_nn_samples(const tina_lights *lights, const tina_ibl *ibl, uint32_t seed, int64_t nsamples,
+                    float r_lo, float r_hi, float p_metal, float specular, float *d_x, float *d_y,
+                    void *stream);
+
+/* One Adam step (b1 0.9, b2 0.999, eps 1e-8; step counts from 1) on samples
+ * [first, first + batch) of d_x/d_y.  d_partial holds ceil(batch / TINA_NN_BLOCK)
+ * rows of TINA_NN_PARAMS floats (workspace).  d_grad (optional) receives the
+ * batch gradient; lr = 0 computes it without updating anything. */
+int tina_mlp_train_step(const float *d_x, const float *d_y, int64_t first, int64_t nsamples, int batch,
+                        float *d_params, float *d_m1, float *d_m2, float *d_partial, float lr, int step,
+                        float *d_grad, void *stream);
+
+/* float32 parameters -> the packed bf16 inference weights (TINA_NN_PACK_BYTES). */
+int tina_mlp_pack(const float *d_params, void *d_packed, void *stream);
+
+/* The packed network on d_x [n][12] -> d_out [n][4] (max(0, colour), error). */
+int tina_mlp_inference(const void *d_packed, const float *d_x, int64_t n, float *d_out, void *stream);
+
+/* tina_shade for the PBR material under IBL (TINA_IBL | TINA_TONEMAP required)
+ * through the network: pixels whose error estimate exceeds `threshold` are
+ * queued and shaded by the analytic kernel in the same call. */
+int tina_shade_nn(tina_ctx *ctx, const tina_mesh *mesh, const tina_material *mat,
+                  const tina_lights *lights, const tina_ibl *ibl, const void *d_packed, float *d_img,
+                  const float bg[3], uint32_t flags, float threshold, void *stream);
+
+/* Copies (on the stream) the number of pixels the last tina_shade_nn of ctx
+ * shaded analytically into d_count (one uint32 of device memory). */
+int tina_nn_fallback_count(tina_ctx *ctx, uint32_t *d_count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

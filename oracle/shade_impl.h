@@ -130,7 +130,10 @@ static inline VEC FN(cooktorrance)(REAL roughness, VEC f0, VEC n, VEC L, VEC V)
     REAL vdf = RC(1) / (((NoV * k) + RC(1)) - k);
     vdf = vdf * (RC(1) / (((NoL * k) + RC(1)) - k));
     vdf = vdf / ((RC(1) * (RC(1) - alpha2)) + (four_pi * alpha2));
-    REAL p5 = FN(r_pow)(RC(1) - VoH, RC(5));
+    /* (1 - VoH)**5 by squaring: ((x*x)*(x*x))*x (how Taichi lowers an integer
+     * power is not pinned here; squaring keeps both restatements identical) */
+    REAL x1 = RC(1) - VoH, x2 = x1 * x1;
+    REAL p5 = (x2 * x2) * x1;
     VEC fdf = FN(vadd)(f0, FN(vsmul)(p5, FN(vsub)(FN(vs)(RC(1)), f0)));
     return FN(vsmul)(ndf, FN(vsmul)(vdf, fdf));
 }

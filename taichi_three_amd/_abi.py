@@ -87,6 +87,7 @@ class IBLTables(ctypes.Structure):
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int
+_i64 = ctypes.c_int64
 EXPORTS = [
     ('tina_version', _i32, []),
     ('tina_last_error', ctypes.c_char_p, [_vp]),
@@ -102,6 +103,17 @@ EXPORTS = [
     ('tina_tonemap_aces', _i32, [_vp, _i32, _vp]),
     ('tina_resolve', _i32, [_vp, _vp, _vp, _vp]),
     ('tina_keys', _i32, [_vp, ctypes.POINTER(_vp)]),
+    ('tina_bake_ibl', _i32, [_vp, _i32, _i32, _vp, _i32, _i32, ctypes.c_float, _i32, _vp]),
+    ('tina_nn_samples', _i32, [ctypes.POINTER(Lights), ctypes.POINTER(IBLTables), _u32, _i64, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),
+    ('tina_mlp_train_step', _i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp, ctypes.c_float, _i32, _vp,
+                                   _vp]),
+    ('tina_mlp_pack', _i32, [_vp, _vp, _vp]),
+    ('tina_mlp_inference', _i32, [_vp, _vp, _i64, _vp, _vp]),
+    ('tina_shade_nn', _i32, [_vp, ctypes.POINTER(Mesh), ctypes.POINTER(Material), ctypes.POINTER(Lights),
+                             ctypes.POINTER(IBLTables), _vp, _vp, ctypes.POINTER(ctypes.c_float * 3), _u32,
+                             ctypes.c_float, _vp]),
+    ('tina_nn_fallback_count', _i32, [_vp, _vp, _vp]),
 ]
 
 _lib = None

@@ -109,7 +109,9 @@ k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, 
         r.a = make_float4(s.bx, s.by, s.cx, s.cy);
         r.b = make_float4(s.bcnx, s.bcny, s.canx, s.cany);
         r.c = make_float4(s.ws0, s.ws1, s.ws2, s.Az);
-        r.d = make_float4(s.Bz, s.Cz, __int_as_float(s.botx | (s.boty << 16)),
+        // bbox corners (16 bits each; screen < 32768 px) and bit 31: well-conditioned
+        r.d = make_float4(s.Bz, s.Cz,
+                          __int_as_float(s.botx | (s.boty << 16) | (face_well_conditioned(s) ? (int)0x80000000 : 0)),
                           __int_as_float(s.topx | (s.topy << 16)));
         recs[f] = r;
         for (int y = ty0; y <= ty1; y++)
@@ -180,6 +182,7 @@ k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __re
   __shared__ unsigned long long tk[kTile * kTile];
   __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
   __shared__ int32_t Lend[kBlock];
+  __shared__ uint16_t Q[kBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
   __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
   const int t = blockIdx.x;
@@ -222,7 +225,8 @@ k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __re
       const uint32_t fid = entries[chunk_off[(size_t)lo * ntiles + t] + (v - pre[lo])];
       const FaceRec r = recs[fid];
       const uint32_t lo16 = (uint32_t)__float_as_int(r.d.z), hi16 = (uint32_t)__float_as_int(r.d.w);
-      const int bx0 = max((int)(lo16 & 0xFFFF), ox), by0 = max((int)(lo16 >> 16), oy);
+      const int wellc = (int)(lo16 >> 31);
+      const int bx0 = max((int)(lo16 & 0xFFFF), ox), by0 = max((int)((lo16 >> 16) & 0x7FFF), oy);
       const int bx1 = min((int)(hi16 & 0xFFFF), ox + tw - 1), by1 = min((int)(hi16 >> 16), oy + th - 1);
       const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
       if (w > 0 && h > 0) {
@@ -231,7 +235,8 @@ k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __re
         R1[tid] = r.b;
         R2[tid] = r.c;
         // Bz, Cz, clipped origin (x | y << 16), width | face id ... packed below
-        R3[tid] = make_float4(r.d.x, r.d.y, __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16)),
+        R3[tid] = make_float4(r.d.x, r.d.y,
+                              __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) | (wellc << 24)),
                               __int_as_float((int)(id_base + fid)));
       }
     }
@@ -239,39 +244,69 @@ k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __re
     Lend[tid] = end;
     const int wtotal = __shfl(end, 63, 64);
     wave_sync_lds();
-    // One fragment: find its face (binary search over the wave's prefix ends), fetch
-    // the face's record, test the pixel, reduce the key in LDS.
-    auto frag = [&](int gI) {
-      int k = 0;
-#pragma unroll
-      for (int step = 32; step > 0; step >>= 1)
-        if (Lend[wbase + k + step - 1] <= gI) k += step;
-      const int j = wbase + k;
-      const int start = k > 0 ? Lend[j - 1] : 0;
-      const float4 r3 = R3[j];
-      const int packed = __float_as_int(r3.z);
-      const int lx0 = packed & 0xFF, ly0 = (packed >> 8) & 0xFF, wj = packed >> 16;
-      const int li = gI - start;
-      const int dy = (int)(((float)li + 0.5f) / (float)wj);  // li / wj, exact (li < 256)
-      const int dx = li - dy * wj;
-      const float4 r0 = R0[j], r1 = R1[j], r2 = R2[j];
-      FaceSetup q;
-      q.bx = r0.x; q.by = r0.y; q.cx = r0.z; q.cy = r0.w;
-      q.bcnx = r1.x; q.bcny = r1.y; q.canx = r1.z; q.cany = r1.w;
-      q.ws0 = r2.x; q.ws1 = r2.y; q.ws2 = r2.z; q.Az = r2.w;
-      q.Bz = r3.x; q.Cz = r3.y;
-      const int lx = lx0 + dx, ly = ly0 + dy;
+    // Cheap pass over every fragment: find its face (binary search over the wave's
+    // prefix ends), evaluate the unscaled edge values, and drop it when a
+    // well-conditioned face rejects it.  Survivors (about a fifth) are compacted
+    // into the wave's LDS queue and the exact test runs on full waves of them.
+    uint16_t* const q = Q + 2 * wbase;
+    int qn = 0;  // uniform: pending fragments in the queue
+    auto exact = [&](int e) {  // exact test + depth + LDS min for queue entry e
+      const int code = q[e];
+      const int j = wbase + (code >> 8), lx = code & 0xF, ly = (code >> 4) & 0xF;
+      const float4 r0 = R0[j], r1 = R1[j], r2 = R2[j], r3 = R3[j];
+      FaceSetup s;
+      s.bx = r0.x; s.by = r0.y; s.cx = r0.z; s.cy = r0.w;
+      s.bcnx = r1.x; s.bcny = r1.y; s.canx = r1.z; s.cany = r1.w;
+      s.ws0 = r2.x; s.ws1 = r2.y; s.ws2 = r2.z; s.Az = r2.w;
+      s.Bz = r3.x; s.Cz = r3.y;
       float w0, w1, w2;
-      if (pixel_cover_fast(q, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1], w0, w1, w2)) {
-        const uint64_t key = pack_key(pixel_depth(q, w0, w1, w2), (uint32_t)__float_as_int(r3.w));
+      if (pixel_cover_fast(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1], w0, w1, w2)) {
+        const uint64_t key = pack_key(pixel_depth(s, w0, w1, w2), (uint32_t)__float_as_int(r3.w));
         atomicMin(&tk[(ly << kTileLog) + lx], (unsigned long long)key);
       }
     };
-    // two independent fragments per lane per step, so their LDS latencies overlap
-    for (int gI = lane; gI < wtotal; gI += 128) {
-      frag(gI);
-      if (gI + 64 < wtotal) frag(gI + 64);
+    for (int g0 = 0; g0 < wtotal; g0 += 64) {
+      const int gI = g0 + lane;
+      bool keep = false;
+      int code = 0;
+      if (gI < wtotal) {
+        int k = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (Lend[wbase + k + step - 1] <= gI) k += step;
+        const int j = wbase + k;
+        const int start = k > 0 ? Lend[j - 1] : 0;
+        const int packed = __float_as_int(R3[j].z);
+        const int lx0 = packed & 0xFF, ly0 = (packed >> 8) & 0xFF, wj = (packed >> 16) & 0xFF;
+        const int li = gI - start;
+        // li / wj: (li + 1/2) / wj is >= 1/32 from an integer (li < 256, wj <= 16),
+        // far more than the error of a multiply by the hardware reciprocal
+        const int dy = (int)(((float)li + 0.5f) * __builtin_amdgcn_rcpf((float)wj));
+        const int dx = li - dy * wj;
+        const int lx = lx0 + dx, ly = ly0 + dy;
+        const float4 r0 = R0[j], r1 = R1[j];
+        FaceSetup s;
+        s.bx = r0.x; s.by = r0.y; s.cx = r0.z; s.cy = r0.w;
+        s.bcnx = r1.x; s.bcny = r1.y; s.canx = r1.z; s.cany = r1.w;
+        keep = !((packed >> 24) &&
+                 pixel_rejects(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1]));
+        code = (k << 8) | (ly << 4) | lx;
+      }
+      const unsigned long long m = __ballot(keep);
+      if (keep) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)code;
+      qn += __popcll(m);
+      wave_sync_lds();
+      if (qn >= 64) {  // a full wave of survivors
+        exact(lane);
+        const int rest = qn - 64;
+        const int moved = lane < rest ? q[64 + lane] : 0;
+        wave_sync_lds();
+        if (lane < rest) q[lane] = (uint16_t)moved;
+        qn = rest;
+        wave_sync_lds();
+      }
     }
+    if (lane < qn) exact(lane);
     wave_sync_lds();
   }
   __syncthreads();

@@ -1,0 +1,568 @@
+// nn_kernels.h -- the learned shading surrogate for Cook-Torrance + IBL.
+//
+// A 12-64-64-64-4 ReLU network replaces the analytic evaluation of a PBR
+// material (tina/matr/material.py:322-362, 701-706) under SkyboxLighting
+// (tina/core/lighting.py:4-21) plus one directional light: outputs 0-2 are the
+// ACES-mapped colour, output 3 estimates that colour's error, and pixels whose
+// estimate exceeds a threshold are shaded analytically instead.
+//
+//   k_nn_samples   training pairs from the analytic shader (the same code path
+//                  as k_shade<PROG_PBR, true>) at hashed random inputs
+//   k_mlp_grad     forward + backward of one minibatch slice per workgroup, in
+//                  float32, with the slice's gradient written to its own row
+//   k_mlp_adam     sums the rows in a fixed order and takes one Adam step
+//   k_mlp_pack     float32 master weights -> bf16 MFMA fragments + f32 biases
+//   k_shade_nn     deferred shading through the network on MFMA
+//                  (v_mfma_f32_16x16x32_bf16), fallback pixels queued
+//   k_shade_list   the analytic shader over the queued pixels
+//
+// Parameter layout (float32, row-major): W1[64][12] b1[64] W2[64][64] b2[64]
+// W3[64][64] b3[64] W4[4][64] b4[4].
+#pragma once
+#include "shade_kernels.h"
+
+namespace tina {
+
+constexpr int kNnIn = 12, kNnHid = 64, kNnOut = 4;
+constexpr int kOffW1 = 0, kOffB1 = 768, kOffW2 = 832, kOffB2 = 4928, kOffW3 = 4992, kOffB3 = 9088;
+constexpr int kOffW4 = 9152, kOffB4 = 9408, kNnParams = 9412;
+constexpr int kNnBatchPerBlock = 64;      // samples per k_mlp_grad workgroup (one wave)
+constexpr float kHuberBeta = 0.01f;
+constexpr float kErrWeight = 1.0f;
+
+// Packed inference weights: 22 A-operand fragments (64 lanes x 8 bf16) --
+// layer 1: 4 row blocks x 1 K block, layers 2 and 3: 4 x 2, layer 4: 1 x 2 --
+// followed by the f32 biases b1, b2, b3 (64 each) and b4 (4).
+constexpr int kNnFrags = 22;
+constexpr int kNnPackBias = kNnFrags * 64 * 8;  // offset of the biases, in bf16 units
+constexpr int kNnPackBytes = kNnFrags * 64 * 16 + (3 * 64 + 4) * 4;
+
+// ---- random inputs -----------------------------------------------------------
+
+// 32-bit integer hash (xorshift-multiply, "lowbias32")
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// uniform in [0, 1): the k-th draw of sample i under seed
+__device__ __forceinline__ float urand(uint32_t seed, uint64_t i, uint32_t k) {
+  const uint32_t h = hash32(seed ^ hash32((uint32_t)i ^ hash32((uint32_t)(i >> 32) + 0x9e3779b9u * (k + 1))));
+  return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ V3 unit_sphere(float u1, float u2) {
+  const float z = 1.0f - 2.0f * u1;
+  const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+  const float phi = 6.2831855f * u2;
+  return v3(r * cosf(phi), r * sinf(phi), z);
+}
+
+// The 12 network inputs: n, R = 2 (n.V) n - V (the view direction mirrored about
+// n), R.L, log(roughness), metallic, albedo.
+__device__ __forceinline__ void nn_features(V3 n, V3 V, V3 L, float r, float m, V3 alb, float* x) {
+  const float t = 2.0f * vdot(n, V);
+  const V3 R = v3(t * n.x - V.x, t * n.y - V.y, t * n.z - V.z);
+  x[0] = n.x; x[1] = n.y; x[2] = n.z;
+  x[3] = R.x; x[4] = R.y; x[5] = R.z;
+  x[6] = vdot(R, L);
+  x[7] = logf(r);
+  x[8] = m;
+  x[9] = alb.x; x[10] = alb.y; x[11] = alb.z;
+}
+
+// FresnelFactor of PBR (material.py:68-83, 701-706): metallic * albedo +
+// (1 - metallic) * 0.16 * specular^2, in the op order of the material program.
+__device__ __forceinline__ V3 pbr_f0(float m, V3 alb, float spec) {
+  const V3 mt = vs(m), sp = vs(spec);
+  return vadd(vmul(mt, alb), vmul(vsmul(0.16f, vsub(vs(1.0f), mt)), vmul(sp, sp)));
+}
+
+// Training pairs.  X[i] = 12 features, Y[i] = (ACES colour, 0).  n uniform on the
+// sphere, V uniform on the hemisphere around n, roughness U[r_lo, r_hi],
+// metallic Bernoulli(p_metal) (uniform when p_metal < 0), albedo U[0,1]^3.
+__global__ void __launch_bounds__(256)
+k_nn_samples(const ShadeArgs a_by_value, uint32_t seed, long nsamples, float r_lo, float r_hi, float p_metal,
+             float spec, float* __restrict__ X, float* __restrict__ Y) {
+  (void)a_by_value;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const ShadeArgs& a = a_by_value;
+#endif
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsamples) return;
+  const V3 n = unit_sphere(urand(seed, i, 0), urand(seed, i, 1));
+  V3 V = unit_sphere(urand(seed, i, 2), urand(seed, i, 3));
+  if (vdot(V, n) < 0.0f) V = vneg(V);
+  const float r = r_lo + (r_hi - r_lo) * urand(seed, i, 4);
+  const float u = urand(seed, i, 5);
+  const float m = p_metal < 0.0f ? u : (u < p_metal ? 1.0f : 0.0f);
+  const V3 alb = v3(urand(seed, i, 6), urand(seed, i, 7), urand(seed, i, 8));
+  const V3 f0 = pbr_f0(m, alb, spec);
+  V3 c = shade_core<PROG_PBR, true>(a, n, vs(0.0f), V, alb, f0, f0, r);
+  c = aces(c);
+  const V3 L = light_at(a.lights, 0).xyz;
+  float x[kNnIn];
+  nn_features(n, V, L, r, m, alb, x);
+#pragma unroll
+  for (int k = 0; k < kNnIn; k++) X[i * kNnIn + k] = x[k];
+  Y[i * 4 + 0] = c.x;
+  Y[i * 4 + 1] = c.y;
+  Y[i * 4 + 2] = c.z;
+  Y[i * 4 + 3] = 0.0f;
+}
+
+// ---- training ------------------------------------------------------------------
+
+// One wave per workgroup, one sample per lane.  Every lane reads a weight at the
+// same address (uniform: scalar loads), the activations of the 64 samples live
+// in padded LDS rows (row stride 65: the lanes of a column access hit distinct
+// banks), and each delta overwrites the activation rows it no longer needs.
+// The workgroup's gradient -- summed over its samples in sample order -- goes
+// to row blockIdx.x of `partial`.
+// Loss per sample: mean_c smoothL1(o_c - y_c) + kErrWeight/2 (o_3 - max_c|o_c - y_c|)^2,
+// averaged over the `batch` samples of the step; the error term trains W4[3], b4[3] only.
+__global__ void __launch_bounds__(64)
+k_mlp_grad(const float* __restrict__ params, const float* __restrict__ X, const float* __restrict__ Y,
+           long first, long nsamples, int batch, float* __restrict__ partial) {
+  __shared__ float sX[64 * 13];
+  __shared__ float sH1[64 * 65], sH2[64 * 65], sH3[64 * 65];
+  __shared__ float sGo[64 * 4];
+  const int t = threadIdx.x;
+  const long s = first + (long)blockIdx.x * kNnBatchPerBlock + t;
+  const bool valid = s < nsamples;
+  float x[kNnIn], y[3];
+#pragma unroll
+  for (int k = 0; k < kNnIn; k++) x[k] = valid ? X[s * kNnIn + k] : 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; c++) y[c] = valid ? Y[s * 4 + c] : 0.0f;
+#pragma unroll
+  for (int k = 0; k < kNnIn; k++) sX[t * 13 + k] = x[k];
+  __syncthreads();
+
+  // forward
+  for (int j = 0; j < kNnHid; j++) {
+    float z = params[kOffB1 + j];
+#pragma unroll
+    for (int k = 0; k < kNnIn; k++) z += params[kOffW1 + j * kNnIn + k] * x[k];
+    sH1[t * 65 + j] = fmaxf(z, 0.0f);
+  }
+  for (int j = 0; j < kNnHid; j++) {
+    float z = params[kOffB2 + j];
+    for (int k = 0; k < kNnHid; k++) z += params[kOffW2 + j * kNnHid + k] * sH1[t * 65 + k];
+    sH2[t * 65 + j] = fmaxf(z, 0.0f);
+  }
+  for (int j = 0; j < kNnHid; j++) {
+    float z = params[kOffB3 + j];
+    for (int k = 0; k < kNnHid; k++) z += params[kOffW3 + j * kNnHid + k] * sH2[t * 65 + k];
+    sH3[t * 65 + j] = fmaxf(z, 0.0f);
+  }
+  float o[kNnOut];
+#pragma unroll
+  for (int c = 0; c < kNnOut; c++) {
+    float z = params[kOffB4 + c];
+    for (int k = 0; k < kNnHid; k++) z += params[kOffW4 + c * kNnHid + k] * sH3[t * 65 + k];
+    o[c] = z;
+  }
+  // output gradient
+  const float inv_b = 1.0f / (float)batch;
+  float e = 0.0f;
+  float go[kNnOut];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float d = o[c] - y[c];
+    e = fmaxf(e, fabsf(d));
+    const float hg = fabsf(d) < kHuberBeta ? d / kHuberBeta : (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f));
+    go[c] = valid ? hg * (inv_b / 3.0f) : 0.0f;
+  }
+  go[3] = valid ? kErrWeight * (o[3] - e) * inv_b : 0.0f;
+#pragma unroll
+  for (int c = 0; c < kNnOut; c++) sGo[t * 4 + c] = go[c];
+  __syncthreads();
+
+  float* P = partial + (size_t)blockIdx.x * kNnParams;
+  // dW4[c][t] = sum_b go[b][c] h3[b][t]; db4
+  {
+    float acc[kNnOut] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int b = 0; b < 64; b++) {
+      const float h = sH3[b * 65 + t];
+#pragma unroll
+      for (int c = 0; c < kNnOut; c++) acc[c] += sGo[b * 4 + c] * h;
+    }
+#pragma unroll
+    for (int c = 0; c < kNnOut; c++) P[kOffW4 + c * kNnHid + t] = acc[c];
+    if (t < kNnOut) {
+      float sb = 0.0f;
+      for (int b = 0; b < 64; b++) sb += sGo[b * 4 + t];
+      P[kOffB4 + t] = sb;
+    }
+  }
+  // dW3[i][t] = sum_b d3[b][i] h2[b][t]; db3[t] = sum_b d3[b][t]
+  auto outer = [&](const float* D, const float* H, int offW, int offB) {
+    float acc[kNnHid];
+#pragma unroll
+    for (int i = 0; i < kNnHid; i++) acc[i] = 0.0f;
+    float sb = 0.0f;
+    for (int b = 0; b < 64; b++) {
+      const float h = H[b * 65 + t];
+      sb += D[b * 65 + t];
+#pragma unroll
+      for (int i = 0; i < kNnHid; i++) acc[i] += D[b * 65 + i] * h;
+    }
+#pragma unroll
+    for (int i = 0; i < kNnHid; i++) P[offW + i * kNnHid + t] = acc[i];
+    P[offB + t] = sb;
+  };
+  __syncthreads();
+  // delta3 = (W4[0:3]^T go) * (h3 > 0) -> this lane's row of sH3
+  for (int j = 0; j < kNnHid; j++) {
+    float z = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) z += params[kOffW4 + c * kNnHid + j] * go[c];
+    sH3[t * 65 + j] = sH3[t * 65 + j] > 0.0f ? z : 0.0f;
+  }
+  __syncthreads();
+  outer(sH3, sH2, kOffW3, kOffB3);
+  __syncthreads();
+  // delta2 = (W3^T d3) * (h2 > 0) -> this lane's row of sH2
+  for (int j = 0; j < kNnHid; j++) {
+    float z = 0.0f;
+    for (int i = 0; i < kNnHid; i++) z += params[kOffW3 + i * kNnHid + j] * sH3[t * 65 + i];
+    sH2[t * 65 + j] = sH2[t * 65 + j] > 0.0f ? z : 0.0f;
+  }
+  __syncthreads();
+  outer(sH2, sH1, kOffW2, kOffB2);
+  __syncthreads();
+  // delta1 = (W2^T d2) * (h1 > 0) -> this lane's row of sH1
+  for (int j = 0; j < kNnHid; j++) {
+    float z = 0.0f;
+    for (int i = 0; i < kNnHid; i++) z += params[kOffW2 + i * kNnHid + j] * sH2[t * 65 + i];
+    sH1[t * 65 + j] = sH1[t * 65 + j] > 0.0f ? z : 0.0f;
+  }
+  __syncthreads();
+  // dW1[t][k] = sum_b d1[b][t] x[b][k]; db1[t]
+  {
+    float acc[kNnIn];
+#pragma unroll
+    for (int k = 0; k < kNnIn; k++) acc[k] = 0.0f;
+    float sb = 0.0f;
+    for (int b = 0; b < 64; b++) {
+      const float d = sH1[b * 65 + t];
+      sb += d;
+#pragma unroll
+      for (int k = 0; k < kNnIn; k++) acc[k] += d * sX[b * 13 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < kNnIn; k++) P[kOffW1 + t * kNnIn + k] = acc[k];
+    P[kOffB1 + t] = sb;
+  }
+}
+
+// g = sum over the `nrows` rows of partial (row order); Adam with bias
+// corrections c1 = 1/(1 - b1^t), c2 = 1/(1 - b2^t) supplied by the host.
+__global__ void __launch_bounds__(256)
+k_mlp_adam(float* __restrict__ params, float* __restrict__ m1, float* __restrict__ m2,
+           const float* __restrict__ partial, int nrows, float lr, float b1, float b2, float eps, float c1,
+           float c2, float* __restrict__ grad_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNnParams) return;
+  float g = 0.0f;
+  for (int r = 0; r < nrows; r++) g += partial[(size_t)r * kNnParams + p];
+  if (grad_out) grad_out[p] = g;
+  if (lr == 0.0f) return;  // gradient only
+  const float m = b1 * m1[p] + (1.0f - b1) * g;
+  const float v = b2 * m2[p] + (1.0f - b2) * (g * g);
+  m1[p] = m;
+  m2[p] = v;
+  const float mh = m * c1, vh = v * c2;
+  params[p] = params[p] - lr * (mh / (sqrtf(vh) + eps));
+}
+
+// ---- inference -------------------------------------------------------------------
+//
+// v_mfma_f32_16x16x32_bf16 operand layout (lane l, group g = l / 16, j = l % 16):
+//   A (16 x 32):  A[j][8g + e], e = 0..7
+//   B (32 x 16):  B[8g + e][j]
+//   D (16 x 16):  D[4g + q][j], q = 0..3
+// Neurons are the M dimension and pixels the N dimension, so a layer's D
+// fragments hold, per lane, neurons 16mb + 4g + q of pixel j.  The next layer
+// takes K block kb from the D fragments of row blocks 2kb and 2kb + 1 as they
+// lie: K slot 8g + e of block kb is neuron 32kb + 4g + e (e < 4) or
+// 32kb + 16 + 4g + (e - 4) (e >= 4), and the weights are packed in that K order.
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// neuron feeding K slot (g, e) of K block kb in layers 2-4
+__device__ __forceinline__ int nn_kperm(int kb, int g, int e) {
+  return 32 * kb + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+}
+
+// One thread per packed element: fragment f, lane l, element e.
+__global__ void __launch_bounds__(256) k_mlp_pack(const float* __restrict__ params, __bf16* __restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < kNnFrags * 64 * 8) {
+    const int f = idx / 512, l = (idx / 8) % 64, e = idx % 8;
+    const int g = l / 16, j = l % 16;
+    float w = 0.0f;
+    if (f < 4) {  // layer 1: rows 16f + j, K = 8g + e < 12
+      const int k = 8 * g + e;
+      if (k < kNnIn) w = params[kOffW1 + (16 * f + j) * kNnIn + k];
+    } else if (f < 20) {  // layers 2, 3: frag = 4 + 8 (layer - 2) + 2 mb + kb
+      const int q = f - 4, layer = q / 8, mb = (q % 8) / 2, kb = q % 2;
+      const int off = layer == 0 ? kOffW2 : kOffW3;
+      w = params[off + (16 * mb + j) * kNnHid + nn_kperm(kb, g, e)];
+    } else {  // layer 4: rows j < 4
+      const int kb = f - 20;
+      if (j < kNnOut) w = params[kOffW4 + j * kNnHid + nn_kperm(kb, g, e)];
+    }
+    out[idx] = (__bf16)w;
+  } else if (idx < kNnFrags * 64 * 8 + 3 * 64 + 4) {
+    const int b = idx - kNnFrags * 64 * 8;
+    float v;
+    if (b < 64) v = params[kOffB1 + b];
+    else if (b < 128) v = params[kOffB2 + b - 64];
+    else if (b < 192) v = params[kOffB3 + b - 128];
+    else v = params[kOffB4 + b - 192];
+    float* bias = (float*)(out + kNnPackBias);
+    bias[b] = v;
+  }
+}
+
+// relu + bf16 of the D fragments of row blocks 2kb, 2kb+1: the B operand of K block kb
+__device__ __forceinline__ bf16x8 nn_next_b(const f32x4& lo, const f32x4& hi) {
+  bf16x8 r;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    r[q] = (__bf16)fmaxf(lo[q], 0.0f);
+    r[4 + q] = (__bf16)fmaxf(hi[q], 0.0f);
+  }
+  return r;
+}
+
+struct NnWeights {
+  bf16x8 w1[4], w2[8], w3[8], w4[2];
+  f32x4 b1[4], b2[4], b3[4], b4;
+};
+
+__device__ __forceinline__ void nn_load_weights(const __bf16* __restrict__ pk, int lane, NnWeights& w) {
+  const bf16x8* fr = (const bf16x8*)pk;
+  const int g = lane / 16;
+#pragma unroll
+  for (int i = 0; i < 4; i++) w.w1[i] = fr[i * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w.w2[i] = fr[(4 + i) * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w.w3[i] = fr[(12 + i) * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 2; i++) w.w4[i] = fr[(20 + i) * 64 + lane];
+  const float* bias = (const float*)(pk + kNnPackBias);
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      w.b1[mb][q] = bias[16 * mb + 4 * g + q];
+      w.b2[mb][q] = bias[64 + 16 * mb + 4 * g + q];
+      w.b3[mb][q] = bias[128 + 16 * mb + 4 * g + q];
+    }
+#pragma unroll
+  for (int q = 0; q < 4; q++) w.b4[q] = (4 * g + q) < kNnOut ? bias[192 + 4 * g + q] : 0.0f;
+}
+
+// The network on 16 pixels: x = B operand of layer 1 (this lane's 8 features of
+// pixel j); returns D of layer 4 (lanes of group 0 hold outputs 0-3 of pixel j).
+__device__ __forceinline__ f32x4 nn_eval16(const NnWeights& w, bf16x8 x) {
+  f32x4 h[4];
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++) h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w1[mb], x, w.b1[mb], 0, 0, 0);
+  bf16x8 k0 = nn_next_b(h[0], h[1]), k1 = nn_next_b(h[2], h[3]);
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++) {
+    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb], k0, w.b2[mb], 0, 0, 0);
+    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb + 1], k1, h[mb], 0, 0, 0);
+  }
+  k0 = nn_next_b(h[0], h[1]);
+  k1 = nn_next_b(h[2], h[3]);
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++) {
+    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb], k0, w.b3[mb], 0, 0, 0);
+    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb + 1], k1, h[mb], 0, 0, 0);
+  }
+  k0 = nn_next_b(h[0], h[1]);
+  k1 = nn_next_b(h[2], h[3]);
+  f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[0], k0, w.b4, 0, 0, 0);
+  o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[1], k1, o, 0, 0, 0);
+  return o;
+}
+
+// Test entry: the network on n rows of features X[n][12] -> O[n][4], same
+// arithmetic as k_shade_nn.  One wave per 64 rows.
+__global__ void __launch_bounds__(64)
+k_mlp_infer(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, float* __restrict__ O) {
+  __shared__ __attribute__((aligned(16))) __bf16 sx[64][16];
+  __shared__ float so[64][4];
+  const int lane = threadIdx.x, g = lane / 16, j = lane % 16;
+  NnWeights w;
+  nn_load_weights(pk, lane, w);
+  const long base = (long)blockIdx.x * 64;
+  const long r = base + lane;
+#pragma unroll
+  for (int k = 0; k < 16; k++) sx[lane][k] = (__bf16)(r < n && k < kNnIn ? X[r * kNnIn + k] : 0.0f);
+  __syncthreads();
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) {
+    bf16x8 x;
+#pragma unroll
+    for (int e = 0; e < 8; e++) x[e] = g < 2 ? sx[16 * nb + j][8 * g + e] : (__bf16)0.0f;
+    const f32x4 o = nn_eval16(w, x);
+    if (g == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) so[16 * nb + j][q] = o[q];
+    }
+  }
+  __syncthreads();
+  if (r < n) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) O[r * 4 + q] = q < 3 ? fmaxf(so[lane][q], 0.0f) : so[lane][q];
+  }
+}
+
+// Deferred shading through the network.  A workgroup owns kShadePix pixels:
+// it paints the empty ones with the background and compacts the covered ones
+// of this object into an LDS list (as k_shade does); then each wave takes the
+// list 64 entries at a time -- fragment and features per lane, the network on
+// four 16-pixel MFMA blocks, the ACES colour stored where the error estimate is
+// at most `threshold` and the pixel appended to `fbq` otherwise.
+// The material must be the PBR program (PROG_PBR) whose f0 slot is the
+// FresnelFactor node; the network's light input is light 0's direction.
+__global__ void __launch_bounds__(kBlock)
+k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const __bf16* __restrict__ pk,
+           float threshold, uint32_t* __restrict__ fbq, uint32_t* __restrict__ fbn, float* __restrict__ img) {
+  (void)a_by_value;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const ShadeArgs& a = a_by_value;
+#endif
+  __shared__ uint32_t s_pix[kShadePix];
+  __shared__ uint32_t s_face[kShadePix];
+  __shared__ uint32_t s_n;
+  __shared__ __attribute__((aligned(16))) __bf16 s_x[kBlock / 64][64][16];
+  __shared__ float s_o[kBlock / 64][64][4];
+  const int tid = threadIdx.x;
+  const int lane = __lane_id(), wv = tid / 64, g = lane / 16, j = lane % 16;
+  const long npix = (long)a.cam.W * a.cam.H;
+  const long base = (long)blockIdx.x * kShadePix;
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int r = 0; r < kShadePix / kBlock; r++) {
+    const long P = base + r * kBlock + tid;
+    bool take = false;
+    uint32_t f = 0;
+    if (P < npix) {
+      const uint64_t k = keys[P];
+      if (key_depth(k) >= TINA_MAXDEPTH) {
+        if (a.flags & TINA_BG) store_rgb(img, P, background(a, (int)(P % a.cam.W), (int)(P / a.cam.W)));
+      } else {
+        f = (uint32_t)k - a.mesh.id_base;
+        take = f < (uint32_t)a.mesh.nfaces;
+      }
+    }
+    const unsigned long long mask = __ballot(take);
+    uint32_t off = 0;
+    if (lane == 0 && mask) off = atomicAdd(&s_n, (uint32_t)__popcll(mask));
+    off = __shfl(off, 0);
+    if (take) {
+      const uint32_t q = off + (uint32_t)__popcll(mask & lt_mask);
+      s_pix[q] = (uint32_t)(P - base);
+      s_face[q] = f;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = s_n;
+  if (n == 0) return;
+  NnWeights w;
+  nn_load_weights(pk, lane, w);
+  const tina_material& m = a.mat;
+  const int fres = m.ops[2].b;                 // FresnelFactor slot: metallic = its first operand
+  const V3 L = light_at(a.lights, 0).xyz;
+  for (uint32_t c0 = wv * 64; c0 < n; c0 += kBlock) {
+    const uint32_t q = c0 + lane;
+    const bool live = q < n;
+    long P = 0;
+    uint32_t f = 0;
+    float x[kNnIn];
+#pragma unroll
+    for (int k = 0; k < kNnIn; k++) x[k] = 0.0f;
+    if (live) {
+      P = base + s_pix[q];
+      f = s_face[q];
+      const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
+      Frag fr;
+      fragment(a, f, px, py, fr);
+      const V3 V = vneg(view_ray(a.cam, px, py));
+      const V3 alb = eval_slot(m, m.ops[1].a, f, fr);
+      const float rough = eval_slot(m, m.ops[2].a, f, fr).x;
+      const float metal = eval_leaf(m, param_at(m, fres).a, f, fr).x;
+      nn_features(fr.nrm, V, L, rough, metal, alb, x);
+    }
+    __bf16* row = s_x[wv][lane];
+#pragma unroll
+    for (int k = 0; k < 16; k++) row[k] = (__bf16)(k < kNnIn ? x[k] : 0.0f);
+    wave_sync_lds();
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      bf16x8 xb;
+      if (g < 2) xb = *(const bf16x8*)&s_x[wv][16 * nb + j][8 * g];
+      else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) xb[e] = (__bf16)0.0f;
+      }
+      const f32x4 o = nn_eval16(w, xb);
+      if (g == 0) *(f32x4*)s_o[wv][16 * nb + j] = o;
+    }
+    wave_sync_lds();
+    if (live) {
+      const f32x4 o = *(const f32x4*)s_o[wv][lane];
+      const bool fall = !(o[3] <= threshold);
+      const unsigned long long fm = __ballot(fall);
+      uint32_t at = 0;
+      if (lane == 0 && fm) at = atomicAdd(fbn, (uint32_t)__popcll(fm));
+      at = __shfl(at, 0);
+      if (fall) fbq[at + (uint32_t)__popcll(fm & lt_mask)] = (uint32_t)P;
+      else store_rgb(img, P, v3(fmaxf(o[0], 0.0f), fmaxf(o[1], 0.0f), fmaxf(o[2], 0.0f)));
+    }
+    wave_sync_lds();
+  }
+}
+
+// The analytic shader over the queued pixels fbq[0 .. *fbn); also clears
+// *fbn_next, the counter the next k_shade_nn launch appends to.
+template <int PROG, bool IBL>
+__global__ void __launch_bounds__(kBlock)
+k_shade_list(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ fbq,
+             const uint32_t* __restrict__ fbn, uint32_t* __restrict__ fbn_next, float* __restrict__ img) {
+  (void)a_by_value;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const ShadeArgs& a = a_by_value;
+#endif
+  if (blockIdx.x == 0 && threadIdx.x == 0) *fbn_next = 0u;
+  const uint32_t cnt = *fbn;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock) {
+    const long P = fbq[i];
+    const uint32_t f = (uint32_t)keys[P] - a.mesh.id_base;
+    const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
+    store_rgb(img, P, shade_pixel<PROG, IBL>(a, f, px, py));
+  }
+}
+
+}  // namespace tina

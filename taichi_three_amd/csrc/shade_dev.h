@@ -104,7 +104,10 @@ __device__ __forceinline__ V3 cooktorrance(float roughness, V3 f0, V3 n, V3 L, V
   float vdf = 1.0f / (((NoV * k) + 1.0f) - k);
   vdf = vdf * (1.0f / (((NoL * k) + 1.0f) - k));
   vdf = vdf / ((1.0f * (1.0f - alpha2)) + ((float)(4.0 * 3.141592653589793) * alpha2));
-  const float p5 = powf(1.0f - VoH, 5.0f);
+  // (1 - VoH)**5 as ((x * x) * (x * x)) * x: the integer power restated by
+  // squaring, as the binary32 oracle does
+  const float x1 = 1.0f - VoH, x2 = x1 * x1;
+  const float p5 = (x2 * x2) * x1;
   const V3 fdf = vadd(f0, vsmul(p5, vsub(vs(1.0f), f0)));
   return vsmul(ndf, vsmul(vdf, fdf));
 }
@@ -430,11 +433,16 @@ __device__ __forceinline__ V3 eval_slot(const tina_material& m, int i, uint32_t 
   return eval_leaf(m, i, f, fr);
 }
 
+// The lighting of a fixed program once its slots are known: albedo `alb`, mix
+// factor `fac`, the Cook-Torrance f0 and the lobe parameter (roughness or
+// shininess), at position `pos` with unit normal n, seen from direction V.
+template <int PROG, bool IBL>
+__device__ __forceinline__ V3 shade_core(const ShadeArgs& a, V3 n, V3 pos, V3 V, V3 alb, V3 fac, V3 f0,
+                                         float lobe_p);
+
 template <int PROG, bool IBL>
 __device__ __forceinline__ V3 shade_fixed(const ShadeArgs& a, uint32_t f, const Frag& fr, V3 V) {
   const tina_material& m = a.mat;
-  const float inv_pi = (float)(1.0 / 3.141592653589793);
-  const V3 n = fr.nrm;
   const V3 alb = eval_slot(m, m.ops[1].a, f, fr);
   V3 fac = vs(0.0f), f0 = vs(0.0f);
   float lobe_p = 0.0f;
@@ -447,6 +455,13 @@ __device__ __forceinline__ V3 shade_fixed(const ShadeArgs& a, uint32_t f, const 
       lobe_p = eval_slot(m, m.ops[2].a, f, fr).x;  // shineness
     }
   }
+  return shade_core<PROG, IBL>(a, fr.nrm, fr.pos, V, alb, fac, f0, lobe_p);
+}
+
+template <int PROG, bool IBL>
+__device__ __forceinline__ V3 shade_core(const ShadeArgs& a, V3 n, V3 pos, V3 V, V3 alb, V3 fac, V3 f0,
+                                         float lobe_p) {
+  const float inv_pi = (float)(1.0 / 3.141592653589793);
   const V3 omf = vsub(vs(1.0f), fac);
   V3 res;
   if constexpr (IBL) {
@@ -467,7 +482,7 @@ __device__ __forceinline__ V3 shade_fixed(const ShadeArgs& a, uint32_t f, const 
     const Light li = light_at(a.lights, l);
     const V3 lxyz = li.xyz;
     const float lw = li.w;
-    V3 ldir = vsub(lxyz, v3(fr.pos.x * lw, fr.pos.y * lw, fr.pos.z * lw));
+    V3 ldir = vsub(lxyz, v3(pos.x * lw, pos.y * lw, pos.z * lw));
     const float dist = sqrtf(vdot(ldir, ldir));
     ldir = vdivs(ldir, dist);
     const float cos_i = vdot(n, ldir);

@@ -159,6 +159,32 @@ __device__ __forceinline__ bool pixel_cover_fast(const FaceSetup& s, float posx,
   return (w0 >= 0.0f) & (w1 >= 0.0f) & (w2 >= 0.0f);
 }
 
+// A face is well-conditioned when its 1/w are in [2**-20, 2**20], its viewport
+// b, c are within 2**20 and its edge coefficients bcn, can within 2**40 (NaN
+// fails every test).  Then, for pixel centres within 2**20, |w_bc|, |w_ca| <= 2**62,
+// every x_i = w_i * ws_i and the sum stay below 2**86 (no overflow), and a pixel
+// with an unscaled edge value w_bc, w_ca or t below -2**-40 is not covered:
+//   * its x_i is below -2**-60, so x_i / sum (|.| >= 2**-146) keeps its sign and
+//     passes ">= 0" only if sum < 0;
+//   * w_bc + w_ca + t = 1 up to rounding, with t = (1 - w_bc) - w_ca: when one of
+//     them is negative another is at least 1/2, its x_j >= 2**-21 and
+//     x_j / sum (|.| >= 2**-107) is negative for sum < 0 -- so the pixel fails.
+__device__ __forceinline__ bool face_well_conditioned(const FaceSetup& s) {
+  const float lo = 9.5367432e-7f, hi = 1048576.0f;  // 2**-20, 2**20
+  const bool ws = (s.ws0 >= lo) & (s.ws0 <= hi) & (s.ws1 >= lo) & (s.ws1 <= hi) & (s.ws2 >= lo) & (s.ws2 <= hi);
+  const float mp = fmaxf(fmaxf(fabsf(s.bx), fabsf(s.by)), fmaxf(fabsf(s.cx), fabsf(s.cy)));
+  const float me = fmaxf(fmaxf(fabsf(s.bcnx), fabsf(s.bcny)), fmaxf(fabsf(s.canx), fabsf(s.cany)));
+  return ws & (mp <= hi) & (me <= 1.0995116e12f);  // 2**40
+}
+
+// The cheap rejection for a well-conditioned face (see above).  True: not covered.
+__device__ __forceinline__ bool pixel_rejects(const FaceSetup& s, float posx, float posy) {
+  const float w_bc = (posx - s.bx) * s.bcny - (posy - s.by) * s.bcnx;
+  const float w_ca = (posx - s.cx) * s.cany - (posy - s.cy) * s.canx;
+  const float t = (1.0f - w_bc) - w_ca;
+  return fminf(fminf(w_bc, w_ca), t) < -9.0949470e-13f;  // -2**-40
+}
+
 // depth = int((wei.x*Av.z + wei.y*Bv.z + wei.z*Cv.z) * 2**30)   (:121-122)
 __device__ __forceinline__ int32_t pixel_depth(const FaceSetup& s, float w0, float w1, float w2) {
   const float d = ((w0 * s.Az) + (w1 * s.Bz)) + (w2 * s.Cz);

@@ -13,6 +13,8 @@
 #include <new>
 
 #include "bin_kernels.h"
+#include "ibl_kernels.h"
+#include "nn_kernels.h"
 #include "shade_kernels.h"
 
 using namespace tina;
@@ -30,6 +32,10 @@ struct tina_ctx {
   uint32_t* entries = nullptr;
   FaceRec* recs = nullptr;
   BinCounters* binctr = nullptr;
+  // learned-shading fallback queue ([H][W] pixel indices) and its two counters
+  uint32_t* fbq = nullptr;
+  uint32_t* fbn = nullptr;
+  int fb_which = 0;
   tina_camera cam{};
   std::string err;
 };
@@ -92,7 +98,8 @@ void free_scratch(tina_ctx* c) {
 bool binned_fits(const tina_ctx* c, int nfaces) {
   const long ntiles = (long)c->tiles_x * c->tiles_y;
   const long groups = (nfaces + kBinFaces - 1) / kBinFaces;
-  return (ntiles + kBinFaces) * 4 <= 65536 && groups <= c->groups_cap && groups <= kMaxBinGroups;
+  return (ntiles + kBinFaces) * 4 <= 65536 && groups <= c->groups_cap && groups <= kMaxBinGroups &&
+         c->W < 32768 && c->H < 32768;  // FaceRec packs bbox corners in 16 bits
 }
 
 int alloc_scratch(tina_ctx* c, int cap) {
@@ -207,19 +214,21 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
   c->cam.W2V[10] = c->cam.V2W[10] = -1.0f;
   c->cam.bias[0] = c->cam.bias[1] = 0.5f;
   auto release = [c]() {
-    if (c->keys) (void)hipFree(c->keys);
-    if (c->ctr) (void)hipFree(c->ctr);
-    if (c->binctr) (void)hipFree(c->binctr);
+    for (void* p : {(void*)c->keys, (void*)c->ctr, (void*)c->binctr, (void*)c->fbq, (void*)c->fbn})
+      if (p) (void)hipFree(p);
     delete c;
   };
   if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
-      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, sizeof(BinCounters)) != hipSuccess) {
+      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, sizeof(BinCounters)) != hipSuccess ||
+      hipMalloc(&c->fbq, sizeof(uint32_t) * (size_t)width * height) != hipSuccess ||
+      hipMalloc(&c->fbn, 2 * sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
     release();
     return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the depth|id buffer failed");
   }
   if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess ||
-      hipMemset(c->binctr, 0, sizeof(BinCounters)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      hipMemset(c->binctr, 0, sizeof(BinCounters)) != hipSuccess || hipMemset(c->fbn, 0, 2 * sizeof(uint32_t)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
     release();
     return fail(nullptr, TINA_ERR_HIP, "counter initialisation failed");
   }
@@ -236,9 +245,8 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
 int tina_ctx_destroy(tina_ctx* c) {
   if (!c) return TINA_OK;
   free_scratch(c);
-  if (c->keys) (void)hipFree(c->keys);
-  if (c->ctr) (void)hipFree(c->ctr);
-  if (c->binctr) (void)hipFree(c->binctr);
+  for (void* p : {(void*)c->keys, (void*)c->ctr, (void*)c->binctr, (void*)c->fbq, (void*)c->fbn})
+    if (p) (void)hipFree(p);
   delete c;
   return TINA_OK;
 }
@@ -348,6 +356,136 @@ int tina_tonemap_aces(float* d_img, int npixels, void* stream) {
   const long n = 3L * npixels;
   hipLaunchKernelGGL(k_aces, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_img, n);
   return hip_check(nullptr, hipGetLastError(), "k_aces");
+}
+
+int tina_bake_ibl(const float* d_env, int ew, int eh, float* d_out, int ow, int oh, float roughness, int nsamples,
+                  void* stream) {
+  if (!d_env || !d_out) return fail(nullptr, TINA_ERR_INVALID, "tina_bake_ibl: map pointer is NULL");
+  if (ew < 2 || eh < 2 || ow < 2 || oh < 2 || ew > 65536 || eh > 65536 || ow > 65536 || oh > 65536)
+    return fail(nullptr, TINA_ERR_INVALID, "tina_bake_ibl: map sizes must lie in [2, 65536]");
+  if (nsamples < 1) return fail(nullptr, TINA_ERR_INVALID, "tina_bake_ibl: nsamples must be positive");
+  if (!(roughness == roughness)) return fail(nullptr, TINA_ERR_INVALID, "tina_bake_ibl: roughness is NaN");
+  const double r = (double)roughness;
+  const double alpha2 = roughness < 0.0f ? -1.0 : fmax(0.0, r * r);
+  const long n = (long)ow * oh;
+  hipLaunchKernelGGL(k_bake_ibl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, d_env, ew, eh,
+                     d_out, ow, oh, alpha2, nsamples);
+  return hip_check(nullptr, hipGetLastError(), "k_bake_ibl");
+}
+
+// ---- learned shading ----------------------------------------------------------
+
+static int check_sun(tina_ctx* c, const tina_lights* lights) {
+  if (!lights || lights->n < 1 || lights->n > TINA_MAX_LIGHTS)
+    return fail(c, TINA_ERR_INVALID, "the surrogate needs the lighting with its directional light 0");
+  if (lights->dirs[0][3] != 0.0f) return fail(c, TINA_ERR_INVALID, "light 0 of the surrogate must be directional");
+  return TINA_OK;
+}
+
+static int check_ibl(tina_ctx* c, const tina_ibl* ibl) {
+  if (!ibl || !ibl->diff || !ibl->spec[0]) return fail(c, TINA_ERR_INVALID, "the surrogate needs the IBL tables");
+  for (int i = 0; i < TINA_IBL_LEVELS; i++)
+    if (!ibl->spec[i] || ibl->spec_w[i] < 1 || ibl->spec_h[i] < 1)
+      return fail(c, TINA_ERR_INVALID, "IBL level missing");
+  return TINA_OK;
+}
+
+int tina_nn_samples(const tina_lights* lights, const tina_ibl* ibl, uint32_t seed, int64_t nsamples, float r_lo,
+                    float r_hi, float p_metal, float specular, float* d_x, float* d_y, void* stream) {
+  int rc = check_sun(nullptr, lights);
+  if (rc) return rc;
+  if ((rc = check_ibl(nullptr, ibl))) return rc;
+  if (!d_x || !d_y) return fail(nullptr, TINA_ERR_INVALID, "tina_nn_samples: output is NULL");
+  if (nsamples < 0) return fail(nullptr, TINA_ERR_INVALID, "tina_nn_samples: nsamples < 0");
+  if (!(r_lo > 0.0f && r_lo <= r_hi && r_hi <= 1.0f))
+    return fail(nullptr, TINA_ERR_INVALID, "tina_nn_samples: roughness range must lie in (0, 1]");
+  if (nsamples == 0) return TINA_OK;
+  ShadeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.lights = *lights;
+  a.ibl = *ibl;
+  a.flags = TINA_IBL;
+  hipLaunchKernelGGL(k_nn_samples, dim3((unsigned)((nsamples + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
+                     seed, (long)nsamples, r_lo, r_hi, p_metal, specular, d_x, d_y);
+  return hip_check(nullptr, hipGetLastError(), "k_nn_samples");
+}
+
+int tina_mlp_train_step(const float* d_x, const float* d_y, int64_t first, int64_t nsamples, int batch,
+                        float* d_params, float* d_m1, float* d_m2, float* d_partial, float lr, int step,
+                        float* d_grad, void* stream) {
+  if (!d_x || !d_y || !d_params || !d_m1 || !d_m2 || !d_partial)
+    return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_train_step: buffer is NULL");
+  if (batch < 1 || first < 0 || first + batch > nsamples)
+    return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_train_step: the batch must lie inside the samples");
+  if (step < 1 || !(lr >= 0.0f)) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_train_step: step >= 1, lr >= 0");
+  const int nblocks = (batch + kNnBatchPerBlock - 1) / kNnBatchPerBlock;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nblocks), dim3(64), 0, (hipStream_t)stream, d_params, d_x, d_y, (long)first,
+                     (long)(first + batch), batch, d_partial);
+  int rc = hip_check(nullptr, hipGetLastError(), "k_mlp_grad");
+  if (rc) return rc;
+  const double b1 = 0.9, b2 = 0.999;
+  const float c1 = (float)(1.0 / (1.0 - pow(b1, step))), c2 = (float)(1.0 / (1.0 - pow(b2, step)));
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kNnParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_params, d_m1,
+                     d_m2, d_partial, nblocks, lr, (float)b1, (float)b2, 1e-8f, c1, c2, d_grad);
+  return hip_check(nullptr, hipGetLastError(), "k_mlp_adam");
+}
+
+int tina_mlp_pack(const float* d_params, void* d_packed, void* stream) {
+  if (!d_params || !d_packed) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_pack: buffer is NULL");
+  const int n = kNnFrags * 64 * 8 + 3 * 64 + 4;
+  hipLaunchKernelGGL(k_mlp_pack, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_params,
+                     (__bf16*)d_packed);
+  return hip_check(nullptr, hipGetLastError(), "k_mlp_pack");
+}
+
+int tina_mlp_inference(const void* d_packed, const float* d_x, int64_t n, float* d_out, void* stream) {
+  if (!d_packed || !d_x || !d_out) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: buffer is NULL");
+  if (n < 0) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: n < 0");
+  if (n == 0) return TINA_OK;
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const __bf16*)d_packed, d_x, (long)n, d_out);
+  return hip_check(nullptr, hipGetLastError(), "k_mlp_infer");
+}
+
+int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,
+                  const tina_ibl* ibl, const void* d_packed, float* d_img, const float bg[3], uint32_t flags,
+                  float threshold, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (!mesh || !d_img || !mesh->verts || !d_packed)
+    return fail(c, TINA_ERR_INVALID, "mesh, verts, image or network is NULL");
+  if (mesh->nfaces < 0) return fail(c, TINA_ERR_INVALID, "nfaces must be >= 0");
+  if ((flags & TINA_SMOOTH) && !mesh->norms) return fail(c, TINA_ERR_INVALID, "TINA_SMOOTH without normals");
+  if ((flags & TINA_VCOLOR) && !mesh->colors) return fail(c, TINA_ERR_INVALID, "TINA_VCOLOR without colours");
+  if ((flags & (TINA_IBL | TINA_TONEMAP)) != (TINA_IBL | TINA_TONEMAP) || (flags & (TINA_NOLIGHT | TINA_TEX)))
+    return fail(c, TINA_ERR_INVALID, "the surrogate shades IBL + lights with ACES, without textures");
+  int rc = check_material(c, mat);
+  if (rc) return rc;
+  if ((rc = check_sun(c, lights))) return rc;
+  if ((rc = check_ibl(c, ibl))) return rc;
+  if (classify_program(*mat, true) != PROG_PBR || mat->params[mat->ops[2].b].kind != TINA_P_FRESNEL ||
+      mat->ops[2].b != mat->ops[3].a)
+    return fail(c, TINA_ERR_INVALID, "the surrogate replaces the PBR material (CookTorrance f0 = FresnelFactor)");
+  ShadeArgs a;
+  rc = make_shade_args(c, mesh, mat, lights, ibl, bg, flags, a);
+  if (rc) return rc;
+  const long npix = (long)c->W * c->H;
+  const int which = c->fb_which;
+  c->fb_which ^= 1;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kShadePix, 0x7fffffff)), dim3(kBlock), 0, s, a, c->keys,
+                     (const __bf16*)d_packed, threshold, c->fbq, c->fbn + which, d_img);
+  TINA_LAUNCH(c, "k_shade_nn");
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade_list<PROG_PBR, true>), dim3(grid_for(npix, kBlock, 2048)), dim3(kBlock), 0,
+                     s, a, c->keys, c->fbq, c->fbn + which, c->fbn + (which ^ 1), d_img);
+  TINA_LAUNCH(c, "k_shade_list");
+  return TINA_OK;
+}
+
+int tina_nn_fallback_count(tina_ctx* c, uint32_t* d_count, void* stream) {
+  if (!c || !d_count) return fail(c, TINA_ERR_INVALID, "ctx or out is NULL");
+  TINA_HIP(c, hipMemcpyAsync(d_count, c->fbn + (c->fb_which ^ 1), sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                             (hipStream_t)stream));
+  return TINA_OK;
 }
 
 int tina_resolve(tina_ctx* c, int32_t* d_id, int32_t* d_depth, void* stream) {

@@ -270,6 +270,18 @@ class Lowered:
         self.uses_tex = uses_tex
         self.ibl_ok = uses_ibl_only_ok
 
+    @property
+    def is_pbr(self):
+        """True for the PBR preset's program [LAMBERT, SCALE, COOKTORRANCE, MIX]
+        whose Cook-Torrance f0 is its FresnelFactor mix factor -- the material the
+        learned surrogate replaces."""
+        s = self.struct
+        ops = [s.ops[i].op for i in range(s.nops)]
+        if ops != [0, 5, 3, 6]:  # TINA_OP_LAMBERT, SCALE, COOKTORRANCE, MIX
+            return False
+        f0 = s.ops[2].b
+        return f0 == s.ops[3].a and s.params[f0].kind == 8  # TINA_P_FRESNEL
+
 
 def lower_material(material, mesh_buffers=None):
     """Walks a material tree into a tina_material program.

@@ -100,8 +100,11 @@ class TriangleRaster:
             f |= _abi.VCOLOR
         return f
 
-    def render_color(self, shader, extra_flags=0, bgcolor=(0.0, 0.0, 0.0)):
-        """render_color (triangle.py:133-153) -> tina_shade for this object's pixels."""
+    def render_color(self, shader, extra_flags=0, bgcolor=(0.0, 0.0, 0.0), neural=None):
+        """render_color (triangle.py:133-153) -> tina_shade for this object's pixels.
+
+        With `neural` (a trained NeuralShading for this shader's lighting) a PBR
+        material is shaded through the network instead (tina_shade_nn)."""
         if self.nfaces == 0 and not (extra_flags & _abi.BG):
             return
         main = shader.main() if isinstance(shader, ShaderGroup) else shader
@@ -127,6 +130,14 @@ class TriangleRaster:
         if self._fb is None:
             mesh.verts = 1  # never dereferenced: no face id can fall in an empty range
         bg = (ctypes.c_float * 3)(*[float(x) for x in np.broadcast_to(np.asarray(bgcolor, dtype=np.float32), 3)])
+        if neural is not None and self._fb is not None and lo.is_pbr and neural.lighting is lighting \
+                and (flags & _abi.TONEMAP) and not (flags & (_abi.TEX | _abi.NOLIGHT)):
+            _abi.check(eng._lib.tina_shade_nn(eng._ctx, ctypes.byref(mesh), ctypes.byref(lo.struct),
+                                              ctypes.byref(lights), ctypes.byref(ibl), _abi.ptr(neural.packed),
+                                              _abi.ptr(img), ctypes.byref(bg), int(flags),
+                                              ctypes.c_float(neural.threshold), _abi.stream_handle()), eng._ctx)
+            shader.post_shade(self, self.id_base, self.nfaces)
+            return
         _abi.check(eng._lib.tina_shade(eng._ctx, ctypes.byref(mesh), ctypes.byref(lo.struct),
                                        ctypes.byref(lights), ctypes.byref(ibl) if ibl is not None else None,
                                        _abi.ptr(img), ctypes.byref(bg), int(flags), _abi.stream_handle()),

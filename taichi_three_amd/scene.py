@@ -41,6 +41,8 @@ class Scene:
         self.engine = Engine(res_x, res_y, maxfaces=options.get('maxfaces', MAX))
         self.res = self.engine.res
         self.ibl = options.get('ibl', False)
+        # a trained NeuralShading for this scene's lighting shades its PBR objects (extension)
+        self.neural = options.get('neural')
         self.tonemap = options.get('tonemap', True)
         self.bgcolor = options.get('bgcolor', 0)
         if not self.ibl:
@@ -53,7 +55,7 @@ class Scene:
             sky = options.get('skybox')
             if sky is None:
                 sky = Skybox.procedural()
-            self.lighting = SkyboxLighting(sky)
+            self.lighting = SkyboxLighting(sky, precision=options.get('ibl_precision', 32))
         import torch
         W, H = int(self.res[0]), int(self.res[1])
         self.image = Field(torch.zeros((H, W, 3), dtype=torch.float32, device=self.engine.device))
@@ -129,7 +131,8 @@ class Scene:
             r = info.raster
             r._fb, r._mesh, r.nfaces, r.id_base = info.fb, obj, info.nfaces, info.id_base
             bg = _abi.BG if first else 0
-            r.render_color(self.shaders[info.material], extra_flags=bg | tm, bgcolor=self.bgcolor)
+            r.render_color(self.shaders[info.material], extra_flags=bg | tm, bgcolor=self.bgcolor,
+                           neural=self.neural)
             first = False
 
     def _fill_background(self, tm):

@@ -97,6 +97,18 @@ def soup_1080p(seed=1, res=(1920, 1080), nfaces=2_000_000):
                     ambient=(0.1, 0.1, 0.1), material='pbr_face')
 
 
+def ibl_1080p(seed=3, res=(1920, 1080), nfaces=1_000_000, sky=(512, 256), precision=32):
+    """The 1080p soup generator under image-based lighting: PBR with per-face
+    roughness/metallic lit by the procedural HDR sky (SkyboxLighting) plus one
+    directional light along the brightest sun."""
+    from .skybox import SUNS
+    w = soup_1080p(seed=seed, res=res, nfaces=nfaces)
+    sun = np.asarray(SUNS[0][0], dtype=np.float64)
+    w.update(name='ibl_1080p', ibl=True, sky=tuple(sky), ibl_precision=int(precision),
+             lights=[dict(dir=tuple(sun / np.linalg.norm(sun)), color=(2.0, 2.0, 2.0))], ambient=(0.0, 0.0, 0.0))
+    return w
+
+
 def make_material(name):
     from . import matr
     if name == 'blinn':
@@ -124,6 +136,9 @@ def build_scene(w, **options):
     from .mesh import SimpleMesh
     W, H = w.res
     T = len(w.verts)
+    if w.get('ibl'):
+        from .skybox import Skybox
+        options = dict(options, ibl=True, skybox=Skybox.procedural(*w.sky), ibl_precision=w.ibl_precision)
     scene = Scene(W, H, smoothing=w.smoothing, maxfaces=T, **options)
     scene.engine.set_camera(w.view, w.proj)
     scene.lighting.clear_lights()

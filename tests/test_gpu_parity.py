@@ -148,6 +148,39 @@ def test_raster_thin_and_screen_filling_faces(gpu):
             assert np.array_equal(depth.cpu().numpy(), od)
 
 
+@pytest.mark.parametrize('seed', [31, 32])
+def test_raster_camera_inside_the_cloud(gpu, seed):
+    """Camera in the middle of the soup: faces next to the eye (1/w far from 1,
+    huge and tiny bboxes, vertices behind the camera) exercise the binned path's
+    non-well-conditioned faces, the giant queue and the clip rule together."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import workloads as wl
+    rng = np.random.default_rng(seed)
+    verts = wl.triangle_soup(rng, 60000, np.log(0.05), 1.0)
+    W, H = 640, 480
+    view = tina.lookat(pos=(0.1, 0.05, 0.0), back=(0.0, 0.0, 0.3))
+    proj = tina.perspective(90, W / H, 0.01, 50)
+    W2V, V2W = tina.camera_matrices(view, proj)
+    cam = orc.camera(W2V, V2W)
+    for culling in (True, False):
+        print(f'case: camera inside the cloud seed={seed} culling={culling}', flush=True)
+        eng = tina.Engine(W, H, maxfaces=len(verts))
+        eng.set_camera(view, proj)
+        r = tina.TriangleRaster(eng, maxfaces=len(verts), culling=culling)
+        m = tina.SimpleMesh(maxfaces=len(verts))
+        m.set_face_verts(verts)
+        eng.clear_depth()
+        r.set_object(m)
+        r.render_occup()
+        ids, depth = eng.resolve()
+        torch.cuda.synchronize()
+        od, oi = orc.raster(W, H, cam, verts, flags=(1 if culling else 0) | 2)
+        assert (oi >= 0).sum() > 1000
+        assert np.array_equal(ids.cpu().numpy(), oi), f'{(ids.cpu().numpy() != oi).sum()} id mismatches'
+        assert np.array_equal(depth.cpu().numpy(), od)
+
+
 def test_multi_object_ties_and_probe(gpu):
     """Two objects with coincident faces: the earlier object wins every tie; the
     probe reports local face indices of the winners."""

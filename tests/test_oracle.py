@@ -380,3 +380,108 @@ def test_lut_corner_fixture_shape():
     c = np.array(d['corner'])
     assert c.shape == (3, 3, 2)
     assert np.all(c[..., 0] < 0.05) and np.all(c[..., 1] > 0.9)
+
+
+# ---------------------------------------------------------------- IBL
+
+def test_lut_corner_table_equals_fixture():
+    """The LUT corner the package ships equals the texels read from assets/lut.jpg."""
+    import json
+    import os
+    from conftest import ROOT
+    from taichi_three_amd.skybox import LUT_CORNER
+    d = json.load(open(os.path.join(ROOT, 'tests', 'golden', 'lut_corner.json')))
+    assert np.array_equal(LUT_CORNER, np.array(d['corner']))
+
+
+def test_skybox_sample_inverts_unmapcoor():
+    """Skybox.sample(unmapcoor(texel)) returns that texel: spheremap and
+    unspheremap (tina/skybox.py:23-35) are inverses on the texel grid."""
+    from taichi_three_amd.skybox import unmapcoor
+    w, h = 48, 24
+    env = np.random.default_rng(0).random((w, h, 3)).astype(np.float32)
+    d = unmapcoor(w, h)
+    for x in range(w):
+        for y in range(1, h - 1):
+            assert np.abs(orc.skybox_sample(env, d[x, y]) - env[x, y]).max() <= 1e-9
+
+
+def test_bakes_of_a_constant_environment_are_that_constant():
+    env = np.full((40, 20, 3), 0.7, np.float32)
+    assert np.abs(orc.bake_lambert(env, 10, 5, 64) - 0.7).max() <= 1e-7
+    assert np.abs(orc.bake_ggx(env, 10, 5, 0.2, 64) - 0.7).max() <= 1e-7
+
+
+def _linear_env(w, h, a):
+    """L(d) = 1 + a.d, tabulated on the equirect grid; exact along y under bilerp."""
+    from taichi_three_amd.skybox import unmapcoor
+    return (1 + unmapcoor(w, h) @ a)[..., None].repeat(3, -1).astype(np.float32)
+
+
+def test_lambert_bake_of_a_linear_environment():
+    """Cosine-weighted hemisphere mean of 1 + a.d around n is 1 + (2/3) a.n."""
+    from taichi_three_amd.skybox import unmapcoor
+    a = np.array([0.3, 0.5, 0.2])
+    out = orc.bake_lambert(_linear_env(256, 128, a), 12, 6, 4096)
+    n = unmapcoor(12, 6)
+    assert np.abs(out[..., 0] - (1 + 2 / 3 * (n @ a))).max() <= 2e-4
+
+
+def test_ggx_bake_of_a_linear_environment():
+    """Around n the azimuths average out, so the u-weighted mean of 1 + a.d is
+    1 + (a.n) sum(u^2) / sum(u) over the GGX cosines u of the sample set."""
+    from taichi_three_amd.skybox import unmapcoor
+    a = np.array([0.3, 0.5, 0.2])
+    ns, r = 512, np.float32(0.65)
+    out = orc.bake_ggx(_linear_env(256, 128, a), 12, 6, r, ns)
+    a2 = float(r) * float(r)
+    u0 = (np.arange(ns) + 0.5) / ns
+    u = np.sqrt((1 - u0) / (1 - u0 * (1 - a2)))
+    n = unmapcoor(12, 6)
+    assert np.abs(out[..., 0] - (1 + (n @ a) * (u * u).sum() / u.sum())).max() <= 1e-3
+
+
+def _host_ibl(c_spec, c_diff=0.25):
+    """Host IBL tables with constant maps: spec level i holds c_spec[i]."""
+    from taichi_three_amd.skybox import make_ibl_struct
+    spec = [np.full((8, 4, 3), c, np.float32) for c in c_spec]
+    diff = np.full((6, 3, 3), c_diff, np.float32)
+    return make_ibl_struct(diff, spec), (spec, diff)
+
+
+def test_cooktorrance_ibl_level_lerp_and_lut_corner():
+    """CookTorrance.sample_ibl (material.py:255-277) at normal incidence: the
+    reflected ray is n, VoH = 1, so AB = bilerp(lut, (1, r)) = lerp over the
+    lut[1][0..1] texels, and the specular maps lerp between the two levels that
+    bracket r: spec(r) * (f0 A + B)."""
+    import taichi_three_amd as tina
+    from taichi_three_amd.matr import lower_material
+    from taichi_three_amd.skybox import LUT_CORNER, ROUGH_LEVELS
+    levels = [1.0, 2.0, 3.0, 5.0, 7.0, 11.0, 13.0]
+    ibl, keep = _host_ibl(levels)
+    n = np.array([0.0, 0.0, 1.0])
+    for r in (0.05, 0.1, 0.3, 0.5, 0.9):
+        f0 = 0.6
+        lo = lower_material(tina.CookTorrance(roughness=r, fresnel=f0))
+        got = orc.shade_point(lo.struct, _abi.Lights(), np.zeros(3), n, n, flags=_abi.IBL | _abi.NOLIGHT, ibl=ibl)
+        k = int(np.searchsorted(ROUGH_LEVELS, r))
+        hi, lo_r = ROUGH_LEVELS[k], ([0] + ROUGH_LEVELS)[k]
+        fac = (hi - r) / (hi - lo_r)
+        spec = levels[k + 1] * (1 - fac) + levels[k] * fac
+        A = LUT_CORNER[1, 0, 0] * (1 - r) + LUT_CORNER[1, 1, 0] * r
+        B = LUT_CORNER[1, 0, 1] * (1 - r) + LUT_CORNER[1, 1, 1] * r
+        assert np.allclose(got, spec * (f0 * A + B), rtol=1e-6), (r, got, spec * (f0 * A + B))
+
+
+def test_lambert_ibl_reads_the_irradiance_map():
+    """Lambert.sample_ibl (material.py:440-442) is diff.sample(n): a constant
+    irradiance map c gives albedo * c, with no ambient term under SkyboxLighting."""
+    import taichi_three_amd as tina
+    from taichi_three_amd.matr import lower_material
+    ibl, keep = _host_ibl([1.0] * 7, c_diff=0.25)
+    lo = lower_material(tina.Lambert() * tina.Const((0.2, 0.4, 0.8)))
+    lt = _abi.Lights()
+    lt.ambient[0] = lt.ambient[1] = lt.ambient[2] = 5.0
+    got = orc.shade_point(lo.struct, lt, np.zeros(3), np.array([0.0, 1.0, 0.0]), np.array([0.0, 0.0, 1.0]),
+                          flags=_abi.IBL | _abi.NOLIGHT, ibl=ibl)
+    assert np.allclose(got, 0.25 * np.array([0.2, 0.4, 0.8]), rtol=1e-12)
