@@ -63,13 +63,92 @@ def cpu_baseline(w, budget_s=12.0):
             'ms_per_frame': round(med * 1e3, 2)}
 
 
+def stage_times(scene, reps):
+    """Mean device ms per render phase, from events recorded on the stream the
+    kernels are launched on (torch's current stream)."""
+    import numpy as np
+    import torch
+    stages, evs = {}, []
+
+    def on_phase(name):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        evs.append((name, e))
+
+    for _ in range(reps):
+        evs.clear()
+        scene.render(on_phase=on_phase)
+        torch.cuda.synchronize()
+        for (n0, e0), (_, e1) in zip(evs[:-1], evs[1:]):
+            stages.setdefault(n0, []).append(e0.elapsed_time(e1))
+    return {k: float(np.mean(v)) for k, v in stages.items()}
+
+
+def traffic_for(stage):
+    """HBM bytes per launch of a stage's kernels, from the committed counter
+    summary profiles/traffic_soup_1080p.json (rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE, scaled by what the counters read for a copy of known size), or
+    None when that summary is absent."""
+    try:
+        with open(os.path.join(ROOT, 'profiles', 'traffic_soup_1080p.json')) as f:
+            return json.load(f)['stages'][stage]['hbm_bytes']
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def psnr(a, b):
+    import numpy as np
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return 10.0 * np.log10(1.0 / max(mse, 1e-30))
+
+
+def neural_vs_analytic(reps, train_steps):
+    """The learned surrogate against the analytic Cook-Torrance + IBL shade at
+    1920x1080 with 1M faces: train on 16M analytic pairs, then time the shade
+    phase both ways and compare the frames."""
+    import torch
+    from taichi_three_amd import workloads as wl
+    from taichi_three_amd.neural import NeuralShading
+    w = wl.ibl_1080p()
+    scene, mesh = wl.build_scene(w)
+    scene.render()
+    torch.cuda.synchronize()
+    ref = scene.img.to_numpy().copy()
+    analytic = stage_times(scene, reps)['shade']
+    ns = NeuralShading(scene.lighting, seed=1)
+    t0 = time.perf_counter()
+    ns.train(nsamples=1 << 24, steps=train_steps, batch=16384)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    scene.neural = ns
+    scene.render()
+    torch.cuda.synchronize()
+    img = scene.img.to_numpy()
+    nfall = ns.fallback_count(scene.engine)
+    neural = stage_times(scene, reps)['shade']
+    ids = scene.engine.ids.to_numpy()
+    covered = int((ids >= 0).sum())
+    return {'workload': 'ibl_1080p: 1920x1080, 1M-face soup, PBR under a 512x256 procedural HDR sky + 1 sun',
+            'analytic_shade_ms': round(analytic, 4), 'neural_shade_ms': round(neural, 4),
+            'speedup': round(analytic / neural, 3), 'psnr_db': round(psnr(img, ref), 2),
+            'fallback_pixels': nfall, 'covered_pixels': covered,
+            'fallback_frac': round(nfall / max(1, covered), 4), 'threshold': ns.threshold,
+            'train': {'samples': 1 << 24, 'steps': train_steps, 'batch': 16384, 'seconds': round(t_train, 2)},
+            'mlp_flop_per_px': 2 * (12 * 64 + 64 * 64 * 2 + 64 * 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-neural', action='store_true')
+    ap.add_argument('--train-steps', type=int, default=20000)
     ap.add_argument('--nfaces', type=int, default=2_000_000)
+    ap.add_argument('--copy-mb', type=int, default=0,
+                    help='first run one device copy of this many MiB (a known-size kernel that '
+                         'calibrates the HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (asked for --gpus {args.gpus}); '
@@ -79,6 +158,13 @@ def main():
     import numpy as np
     import torch
     from taichi_three_amd import workloads as wl
+
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     w = wl.soup_1080p(nfaces=args.nfaces)
     W, H = w.res
@@ -98,22 +184,8 @@ def main():
     fps = args.steps / dt
 
     # per-stage device time with events on the stream the kernels run on
-    stages = {}
-    evs = []
-
-    def on_phase(name):
-        e = torch.cuda.Event(enable_timing=True)
-        e.record()
-        evs.append((name, e))
-
     reps = max(10, args.steps)
-    for _ in range(reps):
-        evs.clear()
-        scene.render(on_phase=on_phase)
-        torch.cuda.synchronize()
-        for (n0, e0), (_, e1) in zip(evs[:-1], evs[1:]):
-            stages.setdefault(n0, []).append(e0.elapsed_time(e1))
-    stage_ms = {k: float(np.mean(v)) for k, v in stages.items()}
+    stage_ms = stage_times(scene, reps)
 
     ids = scene.engine.ids.to_numpy()
     V = int(len(np.unique(ids[ids >= 0])))
@@ -150,11 +222,15 @@ def main():
         'covered_pixels': covered,
         'frame_roofline': {'bytes': frame_bytes, 'achieved_GBs': round(frame_bytes / (ms * 1e-3) / 1e9, 1),
                            'frac': round(frame_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        'roofline': {'bound': 'hbm', 'kernel': {'clear': 'k_clear', 'raster': 'k_raster_small+k_raster_big+k_raster_giant',
+        'roofline': {'bound': 'hbm', 'kernel': {'clear': 'k_clear', 'raster': 'k_bin+k_giant+k_tile',
                                                 'shade': 'k_shade'}[dom],
                      'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None},
+                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic_for(dom)},
     }
+    del scene, mesh
+    torch.cuda.empty_cache()
+    if not args.no_neural:
+        out['neural'] = neural_vs_analytic(max(10, args.steps // 2), args.train_steps)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(w)
         out['speedup_vs_cpu'] = round(fps / out['cpu_baseline']['value'], 1)

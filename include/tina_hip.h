@@ -256,6 +256,43 @@ int tina_shade_nn(tina_ctx *ctx, const tina_mesh *mesh, const tina_material *mat
  * shaded analytically into d_count (one uint32 of device memory). */
 int tina_nn_fallback_count(tina_ctx *ctx, uint32_t *d_count, void *stream);
 
+/* ---- differentiable shading ------------------------------------------------ */
+/* No counterpart in the reference (Tina has no gradients): the PBR material
+ * (tina/matr/material.py:322-362, 387-396, 701-706) under point lights
+ * (tina/core/lighting.py:84-98) with albedo, smooth normal, roughness and
+ * metallic per vertex of an indexed mesh and one scalar intensity per light
+ * (light l's colour is intensity[l] * lights->colors[l]).  Coverage comes from
+ * the packed buffer of the last tina_raster of `verts`, held fixed. */
+typedef struct tina_dmesh {
+    uint64_t verts;     /* [F][3][3] f32: the positions tina_raster was given   */
+    uint64_t faces;     /* [F][3] i32 vertex indices                            */
+    uint64_t normals;   /* [V][3] f32 per-vertex normals (interpolated, then normalised) */
+    uint64_t colors;    /* [V][3] f32 albedo                                    */
+    uint64_t rough;     /* [V] f32                                              */
+    uint64_t metal;     /* [V] f32                                              */
+    uint64_t intensity; /* [lights->n] f32                                      */
+    int32_t nfaces, nverts;
+    uint32_t id_base;
+    float ambient[3];
+    float specular;     /* FresnelFactor specular (0.5 in the PBR preset)       */
+} tina_dmesh;
+
+/* Gradient outputs, accumulated (+=) into caller-zeroed device arrays shaped
+ * like the inputs. */
+typedef struct tina_dgrads {
+    uint64_t colors, normals, rough, metal, intensity;
+} tina_dgrads;
+
+/* Colour of every pixel covered by this mesh into d_img ([H][W][3]); empty
+ * pixels become 0, other objects' pixels are left alone. */
+int tina_dshade_forward(tina_ctx *ctx, const tina_dmesh *mesh, const tina_lights *lights, float *d_img,
+                        void *stream);
+
+/* Adjoint of tina_dshade_forward: d_gimg ([H][W][3]) is dLoss/d(image); the
+ * face weights are recomputed from the id buffer. */
+int tina_dshade_backward(tina_ctx *ctx, const tina_dmesh *mesh, const tina_lights *lights,
+                         const float *d_gimg, const tina_dgrads *grads, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,18 @@ class IBLTables(ctypes.Structure):
                 ('rough_step', ctypes.c_float * 6)]
 
 
+class DMesh(ctypes.Structure):
+    _fields_ = [('verts', ctypes.c_uint64), ('faces', ctypes.c_uint64), ('normals', ctypes.c_uint64),
+                ('colors', ctypes.c_uint64), ('rough', ctypes.c_uint64), ('metal', ctypes.c_uint64),
+                ('intensity', ctypes.c_uint64), ('nfaces', ctypes.c_int32), ('nverts', ctypes.c_int32),
+                ('id_base', ctypes.c_uint32), ('ambient', ctypes.c_float * 3), ('specular', ctypes.c_float)]
+
+
+class DGrads(ctypes.Structure):
+    _fields_ = [('colors', ctypes.c_uint64), ('normals', ctypes.c_uint64), ('rough', ctypes.c_uint64),
+                ('metal', ctypes.c_uint64), ('intensity', ctypes.c_uint64)]
+
+
 # (name, restype, argtypes) of every exported entry point declared in include/tina_hip.h
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -114,6 +126,9 @@ EXPORTS = [
                              ctypes.POINTER(IBLTables), _vp, _vp, ctypes.POINTER(ctypes.c_float * 3), _u32,
                              ctypes.c_float, _vp]),
     ('tina_nn_fallback_count', _i32, [_vp, _vp, _vp]),
+    ('tina_dshade_forward', _i32, [_vp, ctypes.POINTER(DMesh), ctypes.POINTER(Lights), _vp, _vp]),
+    ('tina_dshade_backward', _i32, [_vp, ctypes.POINTER(DMesh), ctypes.POINTER(Lights), _vp,
+                                    ctypes.POINTER(DGrads), _vp]),
 ]
 
 _lib = None

@@ -13,6 +13,7 @@
 #include <new>
 
 #include "bin_kernels.h"
+#include "diff_kernels.h"
 #include "ibl_kernels.h"
 #include "nn_kernels.h"
 #include "shade_kernels.h"
@@ -485,6 +486,67 @@ int tina_nn_fallback_count(tina_ctx* c, uint32_t* d_count, void* stream) {
   if (!c || !d_count) return fail(c, TINA_ERR_INVALID, "ctx or out is NULL");
   TINA_HIP(c, hipMemcpyAsync(d_count, c->fbn + (c->fb_which ^ 1), sizeof(uint32_t), hipMemcpyDeviceToDevice,
                              (hipStream_t)stream));
+  return TINA_OK;
+}
+
+// ---- differentiable shading ------------------------------------------------------
+
+static int make_diff_args(tina_ctx* c, const tina_dmesh* m, const tina_lights* lights, DiffArgs& A) {
+  if (!m || !lights) return fail(c, TINA_ERR_INVALID, "mesh or lights is NULL");
+  if (!m->verts || !m->faces || !m->normals || !m->colors || !m->rough || !m->metal || !m->intensity)
+    return fail(c, TINA_ERR_INVALID, "a differentiable-mesh array is NULL");
+  if (m->nfaces < 0 || m->nverts < 0) return fail(c, TINA_ERR_INVALID, "negative mesh size");
+  if (lights->n < 0 || lights->n > TINA_MAX_LIGHTS) return fail(c, TINA_ERR_CAPACITY, "light count out of range");
+  memset(&A, 0, sizeof(A));
+  A.cam = make_cam(c);
+  A.soup = m->verts;
+  A.faces = m->faces;
+  A.nrm = m->normals;
+  A.col = m->colors;
+  A.rough = m->rough;
+  A.metal = m->metal;
+  A.inten = m->intensity;
+  A.nfaces = m->nfaces;
+  A.id_base = m->id_base;
+  A.nlights = lights->n;
+  for (int l = 0; l < lights->n; l++) {
+    if (lights->dirs[l][3] != 1.0f) return fail(c, TINA_ERR_INVALID, "differentiable shading takes point lights");
+    for (int k = 0; k < 3; k++) {
+      A.lpos[l][k] = lights->dirs[l][k];
+      A.lcol[l][k] = lights->colors[l][k];
+    }
+  }
+  for (int k = 0; k < 3; k++) A.amb[k] = m->ambient[k];
+  A.f0_dielectric = 0.16f * (m->specular * m->specular);
+  return TINA_OK;
+}
+
+int tina_dshade_forward(tina_ctx* c, const tina_dmesh* m, const tina_lights* lights, float* d_img, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (!d_img) return fail(c, TINA_ERR_INVALID, "image is NULL");
+  DiffArgs A;
+  int rc = make_diff_args(c, m, lights, A);
+  if (rc) return rc;
+  const long npix = (long)c->W * c->H;
+  hipLaunchKernelGGL(k_dshade_fwd, dim3(grid_for(npix, kBlock, 0x7fffffff)), dim3(kBlock), 0, (hipStream_t)stream,
+                     A, c->keys, d_img);
+  TINA_LAUNCH(c, "k_dshade_fwd");
+  return TINA_OK;
+}
+
+int tina_dshade_backward(tina_ctx* c, const tina_dmesh* m, const tina_lights* lights, const float* d_gimg,
+                         const tina_dgrads* g, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (!d_gimg || !g || !g->colors || !g->normals || !g->rough || !g->metal || !g->intensity)
+    return fail(c, TINA_ERR_INVALID, "gradient buffer is NULL");
+  DiffArgs A;
+  int rc = make_diff_args(c, m, lights, A);
+  if (rc) return rc;
+  const long npix = (long)c->W * c->H;
+  hipLaunchKernelGGL(k_dshade_bwd, dim3(grid_for(npix, kBlock, 0x7fffffff)), dim3(kBlock), 0, (hipStream_t)stream,
+                     A, c->keys, d_gimg, (float*)g->colors, (float*)g->normals, (float*)g->rough, (float*)g->metal,
+                     (float*)g->intensity);
+  TINA_LAUNCH(c, "k_dshade_bwd");
   return TINA_OK;
 }
 

@@ -146,6 +146,48 @@ class PrimitiveMesh(SimpleMesh):
     def sphere(cls, lons=32, lats=24, rad=1):
         return cls(cls.sphere_faces(lons, lats, rad))
 
+    @staticmethod
+    def icosphere(freq=1, rad=1.0):
+        """Geodesic icosphere of frequency `freq` (extension: the reference has
+        only UV spheres): every icosahedron face split into freq^2 triangles,
+        vertices pushed to the sphere and shared.  Returns (verts [10 freq^2 + 2][3],
+        faces [20 freq^2][3] int32, counter-clockwise seen from outside), float64."""
+        t = (1.0 + 5.0 ** 0.5) / 2.0
+        ico_v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+                          (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], np.float64)
+        ico_f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                 (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                 (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+        n = int(freq)
+        index, verts, faces = {}, [], []
+
+        def vid(p):
+            key = tuple(np.round(p / np.linalg.norm(p), 9))
+            if key not in index:
+                index[key] = len(verts)
+                verts.append(p / np.linalg.norm(p))
+            return index[key]
+
+        for a, b, c in ico_f:
+            A, B, C = ico_v[a], ico_v[b], ico_v[c]
+            grid = {}
+            for i in range(n + 1):
+                for j in range(n + 1 - i):
+                    grid[i, j] = vid(A + (B - A) * (i / n) + (C - A) * (j / n))
+            for i in range(n):
+                for j in range(n - i):
+                    faces.append((grid[i, j], grid[i + 1, j], grid[i, j + 1]))
+                    if i + j + 1 < n:
+                        faces.append((grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]))
+        V = np.array(verts) * rad
+        F = np.array(faces, dtype=np.int32)
+        # orient every face outward
+        P = V[F]
+        nrm = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+        flip = (nrm * P.sum(axis=1)).sum(axis=1) < 0
+        F[flip] = F[flip][:, ::-1]
+        return V, F
+
     @classmethod
     def cylinder(cls, lons=32, lats=4, rad=1, hei=2):
         """Capped cylinder (prim.py:50-94)."""

@@ -42,6 +42,9 @@ def test_struct_layouts_match_header(tmp_path):
         'tina_mesh': (_abi.Mesh, ['verts', 'norms', 'coors', 'colors', 'nfaces', 'id_base']),
         'tina_ibl': (_abi.IBLTables, ['diff', 'diff_w', 'spec', 'spec_w', 'spec_h', 'lut', 'rough_lo',
                                       'rough_hi', 'rough_step']),
+        'tina_dmesh': (_abi.DMesh, ['verts', 'faces', 'normals', 'colors', 'rough', 'metal', 'intensity',
+                                    'nfaces', 'nverts', 'id_base', 'ambient', 'specular']),
+        'tina_dgrads': (_abi.DGrads, ['colors', 'normals', 'rough', 'metal', 'intensity']),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/tina_hip.h"',
              'int main(void) {']
