@@ -432,117 +432,6 @@ k_mlp_infer(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, 
   }
 }
 
-// Deferred shading through the network.  A workgroup owns kShadePix pixels:
-// it paints the empty ones with the background and compacts the covered ones
-// of this object into an LDS list (as k_shade does); then each wave takes the
-// list 64 entries at a time -- fragment and features per lane, the network on
-// four 16-pixel MFMA blocks, the ACES colour stored where the error estimate is
-// at most `threshold` and the pixel appended to `fbq` otherwise.
-// The material must be the PBR program (PROG_PBR) whose f0 slot is the
-// FresnelFactor node; the network's light input is light 0's direction.
-__global__ void __launch_bounds__(kBlock)
-k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const __bf16* __restrict__ pk,
-           float threshold, uint32_t* __restrict__ fbq, uint32_t* __restrict__ fbn, float* __restrict__ img) {
-  (void)a_by_value;
-#if defined(__HIP_DEVICE_COMPILE__)
-  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
-#else
-  const ShadeArgs& a = a_by_value;
-#endif
-  __shared__ uint32_t s_pix[kShadePix];
-  __shared__ uint32_t s_face[kShadePix];
-  __shared__ uint32_t s_n;
-  __shared__ __attribute__((aligned(16))) __bf16 s_x[kBlock / 64][64][16];
-  __shared__ float s_o[kBlock / 64][64][4];
-  const int tid = threadIdx.x;
-  const int lane = __lane_id(), wv = tid / 64, g = lane / 16, j = lane % 16;
-  const long npix = (long)a.cam.W * a.cam.H;
-  const long base = (long)blockIdx.x * kShadePix;
-  if (tid == 0) s_n = 0;
-  __syncthreads();
-  const unsigned long long lt_mask = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int r = 0; r < kShadePix / kBlock; r++) {
-    const long P = base + r * kBlock + tid;
-    bool take = false;
-    uint32_t f = 0;
-    if (P < npix) {
-      const uint64_t k = keys[P];
-      if (key_depth(k) >= TINA_MAXDEPTH) {
-        if (a.flags & TINA_BG) store_rgb(img, P, background(a, (int)(P % a.cam.W), (int)(P / a.cam.W)));
-      } else {
-        f = (uint32_t)k - a.mesh.id_base;
-        take = f < (uint32_t)a.mesh.nfaces;
-      }
-    }
-    const unsigned long long mask = __ballot(take);
-    uint32_t off = 0;
-    if (lane == 0 && mask) off = atomicAdd(&s_n, (uint32_t)__popcll(mask));
-    off = __shfl(off, 0);
-    if (take) {
-      const uint32_t q = off + (uint32_t)__popcll(mask & lt_mask);
-      s_pix[q] = (uint32_t)(P - base);
-      s_face[q] = f;
-    }
-  }
-  __syncthreads();
-  const uint32_t n = s_n;
-  if (n == 0) return;
-  NnWeights w;
-  nn_load_weights(pk, lane, w);
-  const tina_material& m = a.mat;
-  const int fres = m.ops[2].b;                 // FresnelFactor slot: metallic = its first operand
-  const V3 L = light_at(a.lights, 0).xyz;
-  for (uint32_t c0 = wv * 64; c0 < n; c0 += kBlock) {
-    const uint32_t q = c0 + lane;
-    const bool live = q < n;
-    long P = 0;
-    uint32_t f = 0;
-    float x[kNnIn];
-#pragma unroll
-    for (int k = 0; k < kNnIn; k++) x[k] = 0.0f;
-    if (live) {
-      P = base + s_pix[q];
-      f = s_face[q];
-      const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
-      Frag fr;
-      fragment(a, f, px, py, fr);
-      const V3 V = vneg(view_ray(a.cam, px, py));
-      const V3 alb = eval_slot(m, m.ops[1].a, f, fr);
-      const float rough = eval_slot(m, m.ops[2].a, f, fr).x;
-      const float metal = eval_leaf(m, param_at(m, fres).a, f, fr).x;
-      nn_features(fr.nrm, V, L, rough, metal, alb, x);
-    }
-    __bf16* row = s_x[wv][lane];
-#pragma unroll
-    for (int k = 0; k < 16; k++) row[k] = (__bf16)(k < kNnIn ? x[k] : 0.0f);
-    wave_sync_lds();
-#pragma unroll
-    for (int nb = 0; nb < 4; nb++) {
-      bf16x8 xb;
-      if (g < 2) xb = *(const bf16x8*)&s_x[wv][16 * nb + j][8 * g];
-      else {
-#pragma unroll
-        for (int e = 0; e < 8; e++) xb[e] = (__bf16)0.0f;
-      }
-      const f32x4 o = nn_eval16(w, xb);
-      if (g == 0) *(f32x4*)s_o[wv][16 * nb + j] = o;
-    }
-    wave_sync_lds();
-    if (live) {
-      const f32x4 o = *(const f32x4*)s_o[wv][lane];
-      const bool fall = !(o[3] <= threshold);
-      const unsigned long long fm = __ballot(fall);
-      uint32_t at = 0;
-      if (lane == 0 && fm) at = atomicAdd(fbn, (uint32_t)__popcll(fm));
-      at = __shfl(at, 0);
-      if (fall) fbq[at + (uint32_t)__popcll(fm & lt_mask)] = (uint32_t)P;
-      else store_rgb(img, P, v3(fmaxf(o[0], 0.0f), fmaxf(o[1], 0.0f), fmaxf(o[2], 0.0f)));
-    }
-    wave_sync_lds();
-  }
-}
-
 // The analytic shader over the queued pixels fbq[0 .. *fbn); also clears
 // *fbn_next, the counter the next k_shade_nn launch appends to.
 template <int PROG, bool IBL>

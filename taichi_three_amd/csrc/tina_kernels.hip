@@ -16,6 +16,7 @@
 #include "diff_kernels.h"
 #include "ibl_kernels.h"
 #include "nn_kernels.h"
+#include "nn_shade.h"
 #include "shade_kernels.h"
 
 using namespace tina;
@@ -473,10 +474,10 @@ int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, 
   const int which = c->fb_which;
   c->fb_which ^= 1;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kShadePix, 0x7fffffff)), dim3(kBlock), 0, s, a, c->keys,
+  hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kShadePix, kNnShadeGrid)), dim3(kBlock), 0, s, a, c->keys,
                      (const __bf16*)d_packed, threshold, c->fbq, c->fbn + which, d_img);
   TINA_LAUNCH(c, "k_shade_nn");
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade_list<PROG_PBR, true>), dim3(grid_for(npix, kBlock, 2048)), dim3(kBlock), 0,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade_list<PROG_PBR, true>), dim3(grid_for(npix, kBlock, 512)), dim3(kBlock), 0,
                      s, a, c->keys, c->fbq, c->fbn + which, c->fbn + (which ^ 1), d_img);
   TINA_LAUNCH(c, "k_shade_list");
   return TINA_OK;
