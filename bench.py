@@ -137,6 +137,49 @@ def neural_vs_analytic(reps, train_steps):
             'mlp_flop_per_px': 2 * (12 * 64 + 64 * 64 * 2 + 64 * 4)}
 
 
+def batched_views(reps, tune_reps=3):
+    """64 independent scenes (1k-500k faces log-uniform, resolutions from
+    {256^2, 512^2, 1024^2, 1920x1080}, cameras on the radius-3 sphere, seed 2)
+    rendered as one batch: autotuned bin sizes and grids, then timed graph
+    replays; beside it the same scenes rendered one Scene.render() at a time."""
+    import torch
+    from taichi_three_amd import autotune, workloads as wl
+    from taichi_three_amd.batch import BatchRenderer, BatchScenes
+    ws = wl.batch_views()
+    scenes = BatchScenes(ws)
+    t0 = time.perf_counter()
+    best, results = autotune.tune(scenes, reps=tune_reps)
+    t_tune = time.perf_counter() - t0
+    plan = BatchRenderer(scenes, best)
+    ms = autotune.time_plan(plan, reps=reps)
+    plan.close()
+    # the same scenes, one Scene at a time (each its own engine), device time per frame
+    serial = 0.0
+    for w in ws:
+        scene, mesh = wl.build_scene(w)
+        for _ in range(2):
+            scene.render()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            scene.render()
+        e1.record()
+        torch.cuda.synchronize()
+        serial += e0.elapsed_time(e1) / 3
+        del scene, mesh
+    torch.cuda.empty_cache()
+    return {'workload': 'batch_views: 64 scenes, 1k-500k faces (log-uniform), {256^2,512^2,1024^2,1920x1080}, '
+                        'cameras on the radius-3 sphere, seed 2',
+            'scenes': scenes.n, 'faces': scenes.total_faces, 'pixels': scenes.total_pixels,
+            'batch_ms': round(ms, 4), 'serial_scene_render_ms': round(serial, 4),
+            'speedup_vs_serial': round(serial / ms, 3),
+            'scenes_per_s': round(scenes.n / (ms * 1e-3), 1),
+            'mtri_per_s': round(scenes.total_faces / (ms * 1e-3) / 1e6, 1),
+            'mpix_per_s': round(scenes.total_pixels / (ms * 1e-3) / 1e6, 1),
+            'tuning': best.as_dict(), 'tuned_candidates': len(results), 'tune_seconds': round(t_tune, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -144,6 +187,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-neural', action='store_true')
+    ap.add_argument('--no-batch', action='store_true')
     ap.add_argument('--train-steps', type=int, default=20000)
     ap.add_argument('--nfaces', type=int, default=2_000_000)
     ap.add_argument('--copy-mb', type=int, default=0,
@@ -231,6 +275,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_neural:
         out['neural'] = neural_vs_analytic(max(10, args.steps // 2), args.train_steps)
+    if not args.no_batch:
+        out['batch'] = batched_views(max(5, args.steps // 4))
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(w)
         out['speedup_vs_cpu'] = round(fps / out['cpu_baseline']['value'], 1)

@@ -293,6 +293,53 @@ int tina_dshade_forward(tina_ctx *ctx, const tina_dmesh *mesh, const tina_lights
 int tina_dshade_backward(tina_ctx *ctx, const tina_dmesh *mesh, const tina_lights *lights,
                          const float *d_gimg, const tina_dgrads *grads, void *stream);
 
+/* ---- batched multi-view rendering ------------------------------------------ */
+/* No counterpart in the reference (one Scene renders one view at a time,
+ * tina/scene/raster.py:168-214): a batch of independent scenes of any sizes is
+ * cleared, binned, rasterised and shaded by five persistent kernels that walk
+ * the work items of every scene, and the five launches can be captured once
+ * into a HIP graph and replayed with one launch.  Every scene of a batch uses
+ * the same material program (the per-scene parameters may differ). */
+typedef struct tina_scene_desc {
+    int32_t width, height;
+    int32_t nfaces;
+    uint32_t raster_flags;   /* TINA_CULL | TINA_CLIP                            */
+    uint32_t shade_flags;    /* TINA_SMOOTH | TINA_VCOLOR | TINA_TONEMAP | TINA_BG */
+    int32_t pad0;
+    uint64_t img;            /* [H][W][3] f32 output                              */
+    tina_camera cam;
+    tina_mesh mesh;          /* mesh.verts: [F][3][3] f32 positions; id_base 0    */
+    tina_material mat;
+    tina_lights lights;
+    float bg[3];
+    float pad1;
+} tina_scene_desc;
+
+/* The autotuned bin sizes and launch shapes. */
+typedef struct tina_batch_tuning {
+    int32_t bin_faces;   /* faces per binning workgroup: 1024, 2048, 4096 or 8192      */
+    int32_t max_tiles;   /* bbox tiles (16 x 16 px) above which a face takes the giant pass */
+    int32_t grid_bin, grid_tile, grid_shade;  /* persistent workgroups (0: one per work item) */
+} tina_batch_tuning;
+
+typedef struct tina_batch tina_batch;
+
+/* Allocates the batch's key buffers and raster scratch and uploads the scene
+ * table.  The descriptors' device pointers must stay valid for its lifetime. */
+int tina_batch_create(int nscenes, const tina_scene_desc *scenes, const tina_batch_tuning *tune,
+                      tina_batch **out);
+int tina_batch_destroy(tina_batch *batch);
+/* One frame of every scene: five launches on the stream. */
+int tina_batch_render(tina_batch *batch, void *stream);
+/* Captures tina_batch_render into a HIP graph (once) ... */
+int tina_batch_capture(tina_batch *batch, void *stream);
+/* ... and replays it with one graph launch. */
+int tina_batch_launch(tina_batch *batch, void *stream);
+/* Device pointer of scene i's packed depth|id buffer ([H][W] u64). */
+int tina_batch_keys(tina_batch *batch, int scene, uint64_t **d_keys);
+/* tina_resolve over any n packed keys (a batch scene's, for instance). */
+int tina_resolve_keys(const uint64_t *d_keys, int64_t n, int32_t *d_id, int32_t *d_depth, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,18 @@ class DMesh(ctypes.Structure):
                 ('id_base', ctypes.c_uint32), ('ambient', ctypes.c_float * 3), ('specular', ctypes.c_float)]
 
 
+class SceneDesc(ctypes.Structure):
+    _fields_ = [('width', ctypes.c_int32), ('height', ctypes.c_int32), ('nfaces', ctypes.c_int32),
+                ('raster_flags', ctypes.c_uint32), ('shade_flags', ctypes.c_uint32), ('pad0', ctypes.c_int32),
+                ('img', ctypes.c_uint64), ('cam', Camera), ('mesh', Mesh), ('mat', Material), ('lights', Lights),
+                ('bg', ctypes.c_float * 3), ('pad1', ctypes.c_float)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [('bin_faces', ctypes.c_int32), ('max_tiles', ctypes.c_int32), ('grid_bin', ctypes.c_int32),
+                ('grid_tile', ctypes.c_int32), ('grid_shade', ctypes.c_int32)]
+
+
 class DGrads(ctypes.Structure):
     _fields_ = [('colors', ctypes.c_uint64), ('normals', ctypes.c_uint64), ('rough', ctypes.c_uint64),
                 ('metal', ctypes.c_uint64), ('intensity', ctypes.c_uint64)]
@@ -129,6 +141,14 @@ EXPORTS = [
     ('tina_dshade_forward', _i32, [_vp, ctypes.POINTER(DMesh), ctypes.POINTER(Lights), _vp, _vp]),
     ('tina_dshade_backward', _i32, [_vp, ctypes.POINTER(DMesh), ctypes.POINTER(Lights), _vp,
                                     ctypes.POINTER(DGrads), _vp]),
+    ('tina_batch_create', _i32, [_i32, ctypes.POINTER(SceneDesc), ctypes.POINTER(BatchTuning),
+                                 ctypes.POINTER(_vp)]),
+    ('tina_batch_destroy', _i32, [_vp]),
+    ('tina_batch_render', _i32, [_vp, _vp]),
+    ('tina_batch_capture', _i32, [_vp, _vp]),
+    ('tina_batch_launch', _i32, [_vp, _vp]),
+    ('tina_batch_keys', _i32, [_vp, _i32, ctypes.POINTER(_vp)]),
+    ('tina_resolve_keys', _i32, [_vp, _i64, _vp, _vp, _vp]),
 ]
 
 _lib = None

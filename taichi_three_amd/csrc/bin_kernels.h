@@ -46,6 +46,26 @@ struct FaceRec {
   float4 d;  // Bz, Cz, botx | boty << 16, topx | topy << 16
 };
 
+// One raster call on the binned path: its faces, screen, key buffer and the
+// scratch the passes share.  bin_faces (faces per binning workgroup) and
+// max_tiles (bbox tiles above which a face is rasterised as a giant) are the
+// tunable bin sizes; kBinFaces and kBinMaxTiles are the defaults.
+struct BinJob {
+  const float* verts;
+  uint64_t* keys;
+  uint32_t* chunk_off;
+  uint32_t* chunk_cnt;
+  uint32_t* entries;
+  FaceRec* recs;
+  BigRec* giantq;
+  BinCounters* ctr;
+  Cam cam;
+  int T;
+  uint32_t id_base, flags;
+  int tiles_x, ntiles, ngroups;
+  int bin_faces, max_tiles;
+};
+
 // The tiles a face's bbox touches: x0 (12 bits), y0 (12 bits), x span - 1 (4 bits),
 // y span - 1 (4 bits).  0xFFFFFFFF: nothing to bin.
 __device__ __forceinline__ uint32_t pack_span(int tx0, int ty0, int nx, int ny) {
@@ -78,30 +98,34 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(int n, Get v, Put out, 
   return *carry;
 }
 
-__global__ void __launch_bounds__(kBinBlock)
-k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, Cam cam, int tiles_x, int ntiles,
-      int ngroups, uint32_t* __restrict__ chunk_off, uint32_t* __restrict__ chunk_cnt, uint32_t* __restrict__ entries,
-      FaceRec* __restrict__ recs, BigRec* __restrict__ giantq, BinCounters* __restrict__ ctr) {
+// Binning workgroup g of job j (the body of k_bin).  Needs (ntiles + bin_faces)
+// words of dynamic LDS.
+__device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   extern __shared__ uint32_t lds[];
+  const int ntiles = j.ntiles, tiles_x = j.tiles_x, nbf = j.bin_faces;
+  const Cam& cam = j.cam;
   uint32_t* cnt = lds;            // [ntiles] counts, then running offsets
-  uint32_t* span = lds + ntiles;  // [kBinFaces] per-face tile span
+  uint32_t* span = lds + ntiles;  // [bin_faces] per-face tile span
   __shared__ uint32_t s_base, s_carry, s_wsum[kBinBlock / 64];
   const int tid = threadIdx.x;
-  const int g = blockIdx.x;
-  const long f0 = (long)g * kBinFaces;
+  const long f0 = (long)g * nbf;
+  uint32_t* __restrict__ chunk_off = j.chunk_off;
+  uint32_t* __restrict__ chunk_cnt = j.chunk_cnt;
+  uint32_t* __restrict__ entries = j.entries;
+  FaceRec* __restrict__ recs = j.recs;
   for (int t = tid; t < ntiles; t += kBinBlock) cnt[t] = 0;
   __syncthreads();
 
-  for (int i = tid; i < kBinFaces; i += kBinBlock) {
+  for (int i = tid; i < nbf; i += kBinBlock) {
     const long f = f0 + i;
     uint32_t sp = 0xFFFFFFFFu;
     bool giant = false;
     FaceSetup s;
-    if (f < T && setup_face(cam, verts + f * 9, flags, s) && s.topx >= s.botx && s.topy >= s.boty) {
+    if (f < j.T && setup_face(cam, j.verts + f * 9, j.flags, s) && s.topx >= s.botx && s.topy >= s.boty) {
       const int tx0 = s.botx >> kTileLog, tx1 = s.topx >> kTileLog;
       const int ty0 = s.boty >> kTileLog, ty1 = s.topy >> kTileLog;
       const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1;
-      if (nx * ny > kBinMaxTiles || nx > 16 || ny > 16) {
+      if (nx * ny > j.max_tiles || nx > 16 || ny > 16) {
         giant = true;
       } else {
         sp = pack_span(tx0, ty0, nx, ny);
@@ -119,7 +143,7 @@ k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, 
       }
     }
     span[i] = sp;
-    push_rec(giant, s, id_base + (uint32_t)f, giantq, &ctr->giant);
+    push_rec(giant, s, j.id_base + (uint32_t)f, j.giantq, &j.ctr->giant);
   }
   __syncthreads();
 
@@ -130,14 +154,14 @@ k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, 
         cnt[t] = ofs;
       },
       s_wsum, &s_carry);
-  if (tid == 0) s_base = atomicAdd(&ctr->cursor, total);
+  if (tid == 0) s_base = atomicAdd(&j.ctr->cursor, total);
   __syncthreads();
   const uint32_t base = s_base;
   for (int t = tid; t < ntiles; t += kBinBlock) chunk_off[(size_t)g * ntiles + t] = base + cnt[t];
   __syncthreads();
 
   // scatter the face ids (order within a tile is irrelevant: the reduction is a min)
-  for (int i = tid; i < kBinFaces; i += kBinBlock) {
+  for (int i = tid; i < nbf; i += kBinBlock) {
     const uint32_t sp = span[i];
     if (sp == 0xFFFFFFFFu) continue;
     const int tx0 = sp & 0xFFF, ty0 = (sp >> 12) & 0xFFF, nx = ((sp >> 24) & 0xF) + 1, ny = (sp >> 28) + 1;
@@ -152,18 +176,16 @@ k_bin(const float* __restrict__ verts, int T, uint32_t id_base, uint32_t flags, 
 // Faces too large to bin: every lane of the grid takes a slice of each one and
 // reduces into the key buffer with global atomics.  Runs between k_bin and
 // k_tile (k_tile loads the keys after it and resets the queue).
-__global__ void __launch_bounds__(kBlock)
-k_giant(const BigRec* __restrict__ giantq, const BinCounters* __restrict__ ctr, Cam cam,
-        uint64_t* __restrict__ keys) {
-  const uint32_t ng = ctr->giant;
+__device__ __forceinline__ void giant_body(const BinJob& j) {
+  const uint32_t ng = j.ctr->giant;
   const long stride = (long)gridDim.x * kBlock;
   for (uint32_t r = 0; r < ng; r++) {
-    const BigRec rec = giantq[r];
+    const BigRec rec = j.giantq[r];
     const long w = rec.s.topx - rec.s.botx + 1;
     const long area = w * (long)(rec.s.topy - rec.s.boty + 1);
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < area; i += stride) {
       const long dy = i / w;
-      cover_pixel(rec.s, cam, rec.s.botx + (int)(i - dy * w), rec.s.boty + (int)dy, rec.id, keys);
+      cover_pixel(rec.s, j.cam, rec.s.botx + (int)(i - dy * w), rec.s.boty + (int)dy, rec.id, j.keys);
     }
   }
 }
@@ -174,10 +196,8 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __restrict__ chunk_off,
-       const uint32_t* __restrict__ chunk_cnt, const uint32_t* __restrict__ entries,
-       const FaceRec* __restrict__ recs, uint64_t* __restrict__ keys, BinCounters* __restrict__ ctr) {
+// Tile t of job j (the body of k_tile).  Needs (ngroups + 1) words of dynamic LDS.
+__device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
   __shared__ unsigned long long tk[kTile * kTile];
   __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
@@ -185,15 +205,17 @@ k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __re
   __shared__ uint16_t Q[kBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
   __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
-  const int t = blockIdx.x;
+  const Cam& cam = j.cam;
+  const int tiles_x = j.tiles_x, ngroups = j.ngroups;
+  const uint32_t id_base = j.id_base;
+  const uint32_t* __restrict__ chunk_off = j.chunk_off;
+  const uint32_t* __restrict__ chunk_cnt = j.chunk_cnt;
+  const uint32_t* __restrict__ entries = j.entries;
+  const FaceRec* __restrict__ recs = j.recs;
+  uint64_t* __restrict__ keys = j.keys;
   const int tx = t % tiles_x, ty = t / tiles_x;
   const int ox = tx << kTileLog, oy = ty << kTileLog;
-  // every binning workgroup has finished: the next raster call starts from empty queues
-  if (t == 0 && tid == 0) {
-    ctr->cursor = 0;
-    ctr->giant = 0;
-  }
-  const int ntiles = gridDim.x;  // chunk tables are group-major: [group][tile]
+  const int ntiles = j.ntiles;  // chunk tables are group-major: [group][tile]
   const uint32_t total = block_exclusive_scan<kBlock>(
       ngroups, [&](int c) { return chunk_cnt[(size_t)c * ntiles + t]; },
       [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);
@@ -313,6 +335,81 @@ k_tile(uint32_t id_base, Cam cam, int tiles_x, int ngroups, const uint32_t* __re
   for (int i = tid; i < kTile * kTile; i += kBlock) {
     const int lx = i & (kTile - 1), ly = i >> kTileLog;
     if (lx < tw && ly < th) keys[(size_t)(oy + ly) * cam.W + ox + lx] = tk[i];
+  }
+}
+
+// ---- kernels: one raster call -------------------------------------------------
+//
+// The job arrives by value as the first kernel argument and is read in place in
+// the kernarg segment (naming the by-value parameter would copy it to scratch).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TINA_KERNARG_JOB(T, name, byval) \
+  (void)byval;                            \
+  const T& name = *(const T*)__builtin_amdgcn_kernarg_segment_ptr()
+#else
+#define TINA_KERNARG_JOB(T, name, byval) const T& name = byval
+#endif
+
+__global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
+  TINA_KERNARG_JOB(BinJob, j, jv);
+  bin_group(j, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_giant(const BinJob jv) {
+  TINA_KERNARG_JOB(BinJob, j, jv);
+  giant_body(j);
+}
+
+__global__ void __launch_bounds__(kBlock) k_tile(const BinJob jv) {
+  TINA_KERNARG_JOB(BinJob, j, jv);
+  // every binning workgroup has finished: the next raster call starts from empty queues
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    j.ctr->cursor = 0;
+    j.ctr->giant = 0;
+  }
+  tile_body(j, blockIdx.x);
+}
+
+// ---- kernels: a batch of raster calls (independent scenes) ---------------------
+//
+// Persistent workgroups walk the batch's work items (binning groups, tiles) in
+// order; item i belongs to the job s with prefix[s] <= i < prefix[s + 1].  The
+// jobs live in device memory.
+
+__device__ __forceinline__ int batch_job(const int* __restrict__ prefix, int njobs, long item) {
+  int lo = 0, hi = njobs;  // prefix[lo] <= item < prefix[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kBinBlock)
+k_bbin(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
+  const long total = prefix[njobs];
+  for (long item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = batch_job(prefix, njobs, item);
+    bin_group(jobs[s], (int)(item - prefix[s]));
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bgiant(const BinJob* __restrict__ jobs, int njobs) {
+  for (int s = 0; s < njobs; s++) giant_body(jobs[s]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_btile(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
+  if (blockIdx.x == 0 && threadIdx.x < njobs) {
+    jobs[threadIdx.x].ctr->cursor = 0;
+    jobs[threadIdx.x].ctr->giant = 0;
+  }
+  const long total = prefix[njobs];
+  for (long item = blockIdx.x; item < total; item += gridDim.x) {
+    const int s = batch_job(prefix, njobs, item);
+    tile_body(jobs[s], (int)(item - prefix[s]));
+    __syncthreads();
   }
 }
 

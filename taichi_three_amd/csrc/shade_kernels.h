@@ -23,24 +23,17 @@ __device__ __forceinline__ void store_rgb(float* __restrict__ img, long P, V3 c)
   img[3 * P + 2] = c.z;
 }
 
+// Pixels [chunk * kShadePix, (chunk + 1) * kShadePix) of one shade call (the body
+// of k_shade).
 template <int PROG, bool IBL>
-__global__ void __launch_bounds__(kBlock)
-k_shade(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __restrict__ img) {
-  // Read the 2 KB argument block in place (it is the first kernel argument, at
-  // offset 0 of the kernarg segment): naming the by-value parameter would copy it
-  // to per-lane scratch.
-  (void)a_by_value;
-#if defined(__HIP_DEVICE_COMPILE__)
-  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
-#else
-  const ShadeArgs& a = a_by_value;  // the host pass only needs the signature
-#endif
+__device__ __forceinline__ void shade_chunk(const ShadeArgs& a, const uint64_t* __restrict__ keys,
+                                            float* __restrict__ img, long chunk) {
   __shared__ uint32_t s_pix[kShadePix];
   __shared__ uint32_t s_face[kShadePix];
   __shared__ uint32_t s_n;
   const int tid = threadIdx.x;
   const long npix = (long)a.cam.W * a.cam.H;
-  const long base = (long)blockIdx.x * kShadePix;
+  const long base = chunk * kShadePix;
   if (tid == 0) s_n = 0;
   __syncthreads();
   const int lane = __lane_id();
@@ -76,6 +69,46 @@ k_shade(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __
     const long P = base + s_pix[j];
     const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
     store_rgb(img, P, shade_pixel<PROG, IBL>(a, s_face[j], px, py));
+  }
+}
+
+template <int PROG, bool IBL>
+__global__ void __launch_bounds__(kBlock)
+k_shade(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __restrict__ img) {
+  // Read the 2 KB argument block in place (it is the first kernel argument, at
+  // offset 0 of the kernarg segment): naming the by-value parameter would copy it
+  // to per-lane scratch.
+  (void)a_by_value;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const ShadeArgs& a = a_by_value;  // the host pass only needs the signature
+#endif
+  shade_chunk<PROG, IBL>(a, keys, img, blockIdx.x);
+}
+
+// One scene of a batched shade: its arguments, key buffer and image.
+struct BatchShade {
+  ShadeArgs a;
+  const uint64_t* keys;
+  float* img;
+};
+
+// Persistent workgroups over the pixel chunks of every scene of a batch; chunk
+// item belongs to scene s with prefix[s] <= item < prefix[s + 1].
+template <int PROG, bool IBL>
+__global__ void __launch_bounds__(kBlock)
+k_bshade(const BatchShade* __restrict__ scenes, const int* __restrict__ prefix, int nscenes) {
+  const long total = prefix[nscenes];
+  for (long item = blockIdx.x; item < total; item += gridDim.x) {
+    int lo = 0, hi = nscenes;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= item) lo = mid; else hi = mid;
+    }
+    const BatchShade& sc = scenes[lo];
+    shade_chunk<PROG, IBL>(sc.a, sc.keys, sc.img, item - prefix[lo]);
+    __syncthreads();
   }
 }
 

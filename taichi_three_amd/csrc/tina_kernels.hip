@@ -9,8 +9,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
-#include <string>
+#include <algorithm>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "bin_kernels.h"
 #include "diff_kernels.h"
@@ -295,16 +297,29 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
   hipStream_t s = (hipStream_t)stream;
   const Cam cam = make_cam(c);
   if (binned_fits(c, nfaces)) {
-    const int ntiles = c->tiles_x * c->tiles_y;
-    const int ngroups = (nfaces + kBinFaces - 1) / kBinFaces;
-    hipLaunchKernelGGL(k_bin, dim3(ngroups), dim3(kBinBlock), (size_t)(ntiles + kBinFaces) * 4, s, d_verts, nfaces,
-                       id_base, flags, cam, c->tiles_x, ntiles, ngroups, c->chunk_off, c->chunk_cnt, c->entries,
-                       c->recs, c->giantq, c->binctr);
+    BinJob j;
+    j.verts = d_verts;
+    j.keys = c->keys;
+    j.chunk_off = c->chunk_off;
+    j.chunk_cnt = c->chunk_cnt;
+    j.entries = c->entries;
+    j.recs = c->recs;
+    j.giantq = c->giantq;
+    j.ctr = c->binctr;
+    j.cam = cam;
+    j.T = nfaces;
+    j.id_base = id_base;
+    j.flags = flags;
+    j.tiles_x = c->tiles_x;
+    j.ntiles = c->tiles_x * c->tiles_y;
+    j.ngroups = (nfaces + kBinFaces - 1) / kBinFaces;
+    j.bin_faces = kBinFaces;
+    j.max_tiles = kBinMaxTiles;
+    hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + kBinFaces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
-    hipLaunchKernelGGL(k_giant, dim3(kQueueGrid), dim3(kBlock), 0, s, c->giantq, c->binctr, cam, c->keys);
+    hipLaunchKernelGGL(k_giant, dim3(kQueueGrid), dim3(kBlock), 0, s, j);
     TINA_LAUNCH(c, "k_giant");
-    hipLaunchKernelGGL(k_tile, dim3(ntiles), dim3(kBlock), (size_t)(ngroups + 1) * 4, s, id_base, cam, c->tiles_x,
-                       ngroups, c->chunk_off, c->chunk_cnt, c->entries, c->recs, c->keys, c->binctr);
+    hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), (size_t)(j.ngroups + 1) * 4, s, j);
     TINA_LAUNCH(c, "k_tile");
     return TINA_OK;
   }
@@ -549,6 +564,249 @@ int tina_dshade_backward(tina_ctx* c, const tina_dmesh* m, const tina_lights* li
                      (float*)g->intensity);
   TINA_LAUNCH(c, "k_dshade_bwd");
   return TINA_OK;
+}
+
+// ---- batched multi-view rendering ---------------------------------------------------
+
+}  // extern "C"
+
+struct tina_batch {
+  int n = 0;
+  tina_batch_tuning tune{};
+  int prog = 0;
+  long total_pix = 0;
+  int max_ntiles = 0, max_groups = 0;
+  uint64_t* keys = nullptr;           // all scenes' keys, back to back
+  std::vector<void*> allocs;          // per-scene raster scratch
+  BinJob* d_jobs = nullptr;           // [n]
+  BatchShade* d_shade = nullptr;      // [n]
+  int* d_prefix = nullptr;            // [3][n + 1]: binning groups, tiles, shade chunks
+  int totals[3] = {0, 0, 0};
+  std::vector<long> key_off;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+static void batch_free(tina_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void* p : b->allocs) (void)hipFree(p);
+  for (void* p : {(void*)b->keys, (void*)b->d_jobs, (void*)b->d_shade, (void*)b->d_prefix})
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_batch_tuning* tune, tina_batch** out) {
+  if (!out || !scenes || !tune) return fail(nullptr, TINA_ERR_INVALID, "tina_batch_create: NULL argument");
+  *out = nullptr;
+  if (nscenes < 1 || nscenes > kBlock) return fail(nullptr, TINA_ERR_INVALID, "a batch holds 1 to 256 scenes");
+  const int bf = tune->bin_faces;
+  if (!(bf == 1024 || bf == 2048 || bf == 4096 || bf == 8192))
+    return fail(nullptr, TINA_ERR_INVALID, "bin_faces must be 1024, 2048, 4096 or 8192");
+  if (tune->max_tiles < 1 || tune->max_tiles > 256) return fail(nullptr, TINA_ERR_INVALID, "max_tiles out of [1, 256]");
+  if (tune->grid_bin < 0 || tune->grid_tile < 0 || tune->grid_shade < 0)
+    return fail(nullptr, TINA_ERR_INVALID, "grid sizes must be >= 0");
+  tina_batch* b = new (std::nothrow) tina_batch();
+  if (!b) return fail(nullptr, TINA_ERR_ALLOC, "host allocation failed");
+  b->n = nscenes;
+  b->tune = *tune;
+  std::vector<BinJob> jobs(nscenes);
+  std::vector<BatchShade> sh(nscenes);
+  std::vector<int> prefix(3 * (nscenes + 1), 0);
+  b->key_off.resize(nscenes);
+  for (int i = 0; i < nscenes; i++) {
+    const tina_scene_desc& d = scenes[i];
+    if (d.width <= 0 || d.height <= 0 || d.width >= 32768 || d.height >= 32768 || d.nfaces < 0 || !d.mesh.verts ||
+        !d.img || d.mesh.nfaces != d.nfaces || d.mesh.id_base != 0) {
+      batch_free(b);
+      return fail(nullptr, TINA_ERR_INVALID, "scene " + std::to_string(i) + ": malformed descriptor");
+    }
+    int rc = check_material(nullptr, &d.mat);
+    if (rc) {
+      batch_free(b);
+      return rc;
+    }
+    const int prog = classify_program(d.mat, false);
+    if (i == 0) b->prog = prog;
+    if (prog != b->prog) {
+      batch_free(b);
+      return fail(nullptr, TINA_ERR_INVALID, "every scene of a batch needs the same material program");
+    }
+    const int tx = (d.width + kTile - 1) / kTile, ty = (d.height + kTile - 1) / kTile;
+    const int ntiles = tx * ty;
+    const int ngroups = (d.nfaces + bf - 1) / bf;
+    if ((ntiles + bf) * 4 > 65536) {
+      batch_free(b);
+      return fail(nullptr, TINA_ERR_CAPACITY, "scene " + std::to_string(i) + ": screen too large for bin_faces");
+    }
+    b->max_ntiles = std::max(b->max_ntiles, ntiles);
+    b->max_groups = std::max(b->max_groups, ngroups);
+    b->key_off[i] = b->total_pix;
+    b->total_pix += (long)d.width * d.height;
+    const size_t T = (size_t)std::max(d.nfaces, 1);
+    void *co = nullptr, *cc = nullptr, *en = nullptr, *rc_ = nullptr, *gq = nullptr, *ct = nullptr;
+    bool ok = hipMalloc(&co, 4 * (size_t)ntiles * std::max(ngroups, 1)) == hipSuccess &&
+              hipMalloc(&cc, 4 * (size_t)ntiles * std::max(ngroups, 1)) == hipSuccess &&
+              hipMalloc(&en, 4 * T * (size_t)tune->max_tiles) == hipSuccess &&
+              hipMalloc(&rc_, sizeof(FaceRec) * T) == hipSuccess && hipMalloc(&gq, sizeof(BigRec) * T) == hipSuccess &&
+              hipMalloc(&ct, sizeof(BinCounters)) == hipSuccess;
+    for (void* p : {co, cc, en, rc_, gq, ct})
+      if (p) b->allocs.push_back(p);
+    if (!ok || hipMemset(ct, 0, sizeof(BinCounters)) != hipSuccess) {
+      (void)hipGetLastError();
+      batch_free(b);
+      return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the batch scratch failed");
+    }
+    BinJob& j = jobs[i];
+    j.verts = (const float*)(uintptr_t)d.mesh.verts;
+    j.keys = nullptr;  // set below, once the key buffer exists
+    j.chunk_off = (uint32_t*)co;
+    j.chunk_cnt = (uint32_t*)cc;
+    j.entries = (uint32_t*)en;
+    j.recs = (FaceRec*)rc_;
+    j.giantq = (BigRec*)gq;
+    j.ctr = (BinCounters*)ct;
+    memcpy(j.cam.W2V, d.cam.W2V, sizeof(j.cam.W2V));
+    memcpy(j.cam.V2W, d.cam.V2W, sizeof(j.cam.V2W));
+    j.cam.bias[0] = d.cam.bias[0];
+    j.cam.bias[1] = d.cam.bias[1];
+    j.cam.W = d.width;
+    j.cam.H = d.height;
+    j.T = d.nfaces;
+    j.id_base = 0;
+    j.flags = d.raster_flags;
+    j.tiles_x = tx;
+    j.ntiles = ntiles;
+    j.ngroups = ngroups;
+    j.bin_faces = bf;
+    j.max_tiles = tune->max_tiles;
+    ShadeArgs& a = sh[i].a;
+    memset(&a, 0, sizeof(a));
+    a.mesh = d.mesh;
+    a.mat = d.mat;
+    a.lights = d.lights;
+    a.cam = j.cam;
+    for (int k = 0; k < 3; k++) a.bg[k] = d.bg[k];
+    a.flags = d.shade_flags & ~(TINA_IBL | TINA_NOLIGHT);
+    if (d.lights.n < 0 || d.lights.n > TINA_MAX_LIGHTS) {
+      batch_free(b);
+      return fail(nullptr, TINA_ERR_CAPACITY, "light count out of range");
+    }
+    sh[i].img = (float*)(uintptr_t)d.img;
+    prefix[0 * (nscenes + 1) + i + 1] = prefix[0 * (nscenes + 1) + i] + ngroups;
+    prefix[1 * (nscenes + 1) + i + 1] = prefix[1 * (nscenes + 1) + i] + ntiles;
+    prefix[2 * (nscenes + 1) + i + 1] =
+        prefix[2 * (nscenes + 1) + i] + (int)(((long)d.width * d.height + kShadePix - 1) / kShadePix);
+  }
+  for (int k = 0; k < 3; k++) b->totals[k] = prefix[k * (nscenes + 1) + nscenes];
+  if (hipMalloc(&b->keys, sizeof(uint64_t) * (size_t)b->total_pix) != hipSuccess ||
+      hipMalloc(&b->d_jobs, sizeof(BinJob) * nscenes) != hipSuccess ||
+      hipMalloc(&b->d_shade, sizeof(BatchShade) * nscenes) != hipSuccess ||
+      hipMalloc(&b->d_prefix, sizeof(int) * prefix.size()) != hipSuccess) {
+    (void)hipGetLastError();
+    batch_free(b);
+    return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the batch tables failed");
+  }
+  for (int i = 0; i < nscenes; i++) {
+    jobs[i].keys = b->keys + b->key_off[i];
+    sh[i].keys = b->keys + b->key_off[i];
+  }
+  if (hipMemcpy(b->d_jobs, jobs.data(), sizeof(BinJob) * nscenes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_shade, sh.data(), sizeof(BatchShade) * nscenes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_prefix, prefix.data(), sizeof(int) * prefix.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    batch_free(b);
+    return fail(nullptr, TINA_ERR_HIP, "upload of the batch tables failed");
+  }
+  *out = b;
+  return TINA_OK;
+}
+
+int tina_batch_destroy(tina_batch* b) {
+  batch_free(b);
+  return TINA_OK;
+}
+
+static unsigned batch_grid(int want, int items) {
+  const int g = want > 0 ? std::min(want, items) : items;
+  return (unsigned)std::max(g, 1);
+}
+
+int tina_batch_render(tina_batch* b, void* stream) {
+  if (!b) return fail(nullptr, TINA_ERR_INVALID, "batch is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  const int n = b->n;
+  hipLaunchKernelGGL(k_clear, dim3(grid_for(b->total_pix / 2 + 1, kBlock, 8192)), dim3(kBlock), 0, s, b->keys,
+                     b->total_pix);
+  int rc = hip_check(nullptr, hipGetLastError(), "k_clear");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bbin, dim3(batch_grid(b->tune.grid_bin, b->totals[0])), dim3(kBinBlock),
+                     (size_t)(b->max_ntiles + b->tune.bin_faces) * 4, s, b->d_jobs, b->d_prefix, n);
+  if ((rc = hip_check(nullptr, hipGetLastError(), "k_bbin"))) return rc;
+  hipLaunchKernelGGL(k_bgiant, dim3(kQueueGrid), dim3(kBlock), 0, s, b->d_jobs, n);
+  if ((rc = hip_check(nullptr, hipGetLastError(), "k_bgiant"))) return rc;
+  hipLaunchKernelGGL(k_btile, dim3(batch_grid(b->tune.grid_tile, b->totals[1])), dim3(kBlock),
+                     (size_t)(b->max_groups + 1) * 4, s, b->d_jobs, b->d_prefix + (n + 1), n);
+  if ((rc = hip_check(nullptr, hipGetLastError(), "k_btile"))) return rc;
+  const unsigned gs = batch_grid(b->tune.grid_shade, b->totals[2]);
+  const int* pre = b->d_prefix + 2 * (n + 1);
+  switch (b->prog) {
+    case PROG_DIFFUSE:
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bshade<PROG_DIFFUSE, false>), dim3(gs), dim3(kBlock), 0, s, b->d_shade, pre, n);
+      break;
+    case PROG_CLASSIC:
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bshade<PROG_CLASSIC, false>), dim3(gs), dim3(kBlock), 0, s, b->d_shade, pre, n);
+      break;
+    case PROG_BLINN:
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bshade<PROG_BLINN, false>), dim3(gs), dim3(kBlock), 0, s, b->d_shade, pre, n);
+      break;
+    case PROG_PBR:
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bshade<PROG_PBR, false>), dim3(gs), dim3(kBlock), 0, s, b->d_shade, pre, n);
+      break;
+    default:
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bshade<PROG_GENERIC, false>), dim3(gs), dim3(kBlock), 0, s, b->d_shade, pre, n);
+      break;
+  }
+  return hip_check(nullptr, hipGetLastError(), "k_bshade");
+}
+
+int tina_batch_capture(tina_batch* b, void* stream) {
+  if (!b) return fail(nullptr, TINA_ERR_INVALID, "batch is NULL");
+  if (b->exec) return TINA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = hip_check(nullptr, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (rc) return rc;
+  const int rrc = tina_batch_render(b, stream);
+  hipGraph_t g = nullptr;
+  rc = hip_check(nullptr, hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+  if (rrc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rrc;
+  }
+  if (rc) return rc;
+  b->graph = g;
+  return hip_check(nullptr, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+int tina_batch_launch(tina_batch* b, void* stream) {
+  if (!b || !b->exec) return fail(nullptr, TINA_ERR_INVALID, "batch is NULL or not captured");
+  return hip_check(nullptr, hipGraphLaunch(b->exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int tina_batch_keys(tina_batch* b, int scene, uint64_t** d_keys) {
+  if (!b || !d_keys || scene < 0 || scene >= b->n) return fail(nullptr, TINA_ERR_INVALID, "bad batch or scene");
+  *d_keys = b->keys + b->key_off[scene];
+  return TINA_OK;
+}
+
+int tina_resolve_keys(const uint64_t* d_keys, int64_t n, int32_t* d_id, int32_t* d_depth, void* stream) {
+  if (!d_keys || n < 0) return fail(nullptr, TINA_ERR_INVALID, "tina_resolve_keys: keys NULL or n < 0");
+  if (n == 0 || (!d_id && !d_depth)) return TINA_OK;
+  hipLaunchKernelGGL(k_resolve, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_keys, d_id,
+                     d_depth, (long)n);
+  return hip_check(nullptr, hipGetLastError(), "k_resolve");
 }
 
 int tina_resolve(tina_ctx* c, int32_t* d_id, int32_t* d_depth, void* stream) {

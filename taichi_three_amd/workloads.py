@@ -109,6 +109,29 @@ def ibl_1080p(seed=3, res=(1920, 1080), nfaces=1_000_000, sky=(512, 256), precis
     return w
 
 
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
+
+
+def batch_views(n=64, seed=2, faces=(1000, 500_000), resolutions=BATCH_RESOLUTIONS):
+    """n independent scenes of the 1080p soup generator: face counts log-uniform
+    in `faces`, resolution drawn uniformly from `resolutions`, camera on the
+    radius-3 sphere looking at the origin (up +y)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        T = int(round(np.exp(rng.uniform(np.log(faces[0]), np.log(faces[1])))))
+        res = tuple(resolutions[int(rng.integers(len(resolutions)))])
+        w = soup_1080p(seed=int(rng.integers(2 ** 31)), res=res, nfaces=T)
+        d = rng.normal(size=3)
+        d = 3.0 * d / np.linalg.norm(d)
+        view = lookat(pos=(0, 0, 0), back=tuple(d))
+        proj = perspective(60, res[0] / res[1], 0.05, 500)
+        W2V, V2W = camera_matrices(view, proj)
+        w.update(name='batch_view', view=view, proj=proj, W2V=W2V, V2W=V2W)
+        out.append(w)
+    return out
+
+
 def make_material(name):
     from . import matr
     if name == 'blinn':

@@ -45,6 +45,9 @@ def test_struct_layouts_match_header(tmp_path):
         'tina_dmesh': (_abi.DMesh, ['verts', 'faces', 'normals', 'colors', 'rough', 'metal', 'intensity',
                                     'nfaces', 'nverts', 'id_base', 'ambient', 'specular']),
         'tina_dgrads': (_abi.DGrads, ['colors', 'normals', 'rough', 'metal', 'intensity']),
+        'tina_scene_desc': (_abi.SceneDesc, ['width', 'nfaces', 'shade_flags', 'img', 'cam', 'mesh', 'mat', 'lights',
+                                             'bg']),
+        'tina_batch_tuning': (_abi.BatchTuning, ['bin_faces', 'max_tiles', 'grid_bin', 'grid_tile', 'grid_shade']),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/tina_hip.h"',
              'int main(void) {']
