@@ -81,7 +81,7 @@ def stage_times(scene, reps):
         torch.cuda.synchronize()
         for (n0, e0), (_, e1) in zip(evs[:-1], evs[1:]):
             stages.setdefault(n0, []).append(e0.elapsed_time(e1))
-    return {k: float(np.mean(v)) for k, v in stages.items()}
+    return {k: float(np.median(v)) for k, v in stages.items()}
 
 
 def traffic_for(stage):

@@ -101,8 +101,10 @@ void free_scratch(tina_ctx* c) {
 // The binned path needs its per-tile counters and the group's face spans in LDS.
 bool binned_fits(const tina_ctx* c, int nfaces) {
   const long ntiles = (long)c->tiles_x * c->tiles_y;
-  const long groups = (nfaces + kBinFaces - 1) / kBinFaces;
-  return (ntiles + kBinFaces) * 4 <= 65536 && groups <= c->groups_cap && groups <= kMaxBinGroups &&
+  const int bf = bin_faces_for(ntiles);
+  if (bf == 0) return false;
+  const long groups = (nfaces + bf - 1) / bf;
+  return groups <= c->groups_cap && groups <= kMaxBinGroups &&
          c->W < 32768 && c->H < 32768;  // FaceRec packs bbox corners in 16 bits
 }
 
@@ -110,10 +112,11 @@ int alloc_scratch(tina_ctx* c, int cap) {
   free_scratch(c);
   const size_t n = (size_t)(cap > 0 ? cap : 1);
   const size_t ntiles = (size_t)c->tiles_x * c->tiles_y;
-  const size_t groups = (n + kBinFaces - 1) / kBinFaces;
+  const int bf = bin_faces_for((long)ntiles);
+  const size_t groups = bf ? (n + bf - 1) / bf : 0;
   bool ok = hipMalloc(&c->bigq, sizeof(BigRec) * n) == hipSuccess &&
             hipMalloc(&c->giantq, sizeof(BigRec) * n) == hipSuccess;
-  if (ok && (ntiles + kBinFaces) * 4 <= 65536 && groups <= (size_t)kMaxBinGroups) {
+  if (ok && bf && groups <= (size_t)kMaxBinGroups) {
     ok = hipMalloc(&c->chunk_off, 4 * ntiles * groups) == hipSuccess &&
          hipMalloc(&c->chunk_cnt, 4 * ntiles * groups) == hipSuccess &&
          hipMalloc(&c->entries, 4 * n * kBinMaxTiles) == hipSuccess &&
@@ -312,10 +315,10 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.flags = flags;
     j.tiles_x = c->tiles_x;
     j.ntiles = c->tiles_x * c->tiles_y;
-    j.ngroups = (nfaces + kBinFaces - 1) / kBinFaces;
-    j.bin_faces = kBinFaces;
+    j.bin_faces = bin_faces_for(j.ntiles);
+    j.ngroups = (nfaces + j.bin_faces - 1) / j.bin_faces;
     j.max_tiles = kBinMaxTiles;
-    hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + kBinFaces) * 4, s, j);
+    hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + j.bin_faces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
     hipLaunchKernelGGL(k_giant, dim3(kQueueGrid), dim3(kBlock), 0, s, j);
     TINA_LAUNCH(c, "k_giant");
