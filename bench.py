@@ -195,8 +195,9 @@ def main():
                          'calibrates the HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f'bench.py measures one GPU only (asked for --gpus {args.gpus}); '
-              'the multi-GPU sharding is described in DESIGN.md, not run.', file=sys.stderr)
+        print(f'bench.py measures one GPU only (asked for --gpus {args.gpus}): the path runs on a '
+              'single GPU; sharding over several GPUs is a design, not something this script runs.',
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -235,7 +236,7 @@ def main():
     V = int(len(np.unique(ids[ids >= 0])))
     covered = int((ids >= 0).sum())
     npix = W * H
-    # algorithmic bytes per launch ("Measurement")
+    # algorithmic bytes per launch: compulsory reads and writes of each stage
     bytes_stage = {
         'clear': 8 * npix,                              # write the packed keys
         'raster': 36 * T,                               # read every face's positions

@@ -97,7 +97,7 @@ class Param(Node):
         self.initial = initial
 
     def make_slider(self, gui, title, min=0, max=1, step=0.01):
-        raise NotImplementedError('GUI sliders are outside this build')
+        raise NotImplementedError('GUI sliders are not provided by taichi_three_amd')
 
 
 class Input(Node):

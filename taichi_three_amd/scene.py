@@ -36,7 +36,7 @@ class Scene:
     def __init__(self, res_x=512, res_y=None, **options):
         for k in _UNSUPPORTED:
             if options.get(k, False):
-                raise NotImplementedError(f'Scene({k}=True): post-processing passes are outside this build')
+                raise NotImplementedError(f'Scene({k}=True): post-processing passes are not provided by taichi_three_amd')
         self.options = options
         self.engine = Engine(res_x, res_y, maxfaces=options.get('maxfaces', MAX))
         self.res = self.engine.res
@@ -96,7 +96,7 @@ class Scene:
         self.engine.set_camera(view, proj)
 
     def input(self, gui):
-        raise NotImplementedError('GUI camera control is outside this build; use scene.set_camera(view, proj)')
+        raise NotImplementedError('GUI camera control is not provided by taichi_three_amd; use scene.set_camera(view, proj)')
 
     def render(self, on_phase=None):
         """clear -> coverage of every object -> shade per object (+bg, +ACES).
