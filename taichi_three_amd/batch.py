@@ -67,6 +67,11 @@ class BatchScenes:
         self.images, self.res, self.nfaces = [], [], []
         for i, w in enumerate(workloads):
             W, H = int(w['res'][0]), int(w['res'][1])
+            T = len(w['verts'])
+            if T == 0:  # one dummy face keeps every pointer valid; nfaces stays 0
+                w = dict(w, verts=np.zeros((1, 3, 3), np.float32), colors=np.zeros((1, 3, 3), np.float32),
+                         norms=None, attrs={k: np.zeros((1,) + np.shape(v)[1:], np.float32)
+                                            for k, v in w.get('attrs', {}).items()})
             verts = torch.from_numpy(np.ascontiguousarray(w['verts'], np.float32)).to(self.dev)
             colors = torch.from_numpy(np.ascontiguousarray(w['colors'], np.float32)).to(self.dev)
             norms = None
@@ -81,7 +86,7 @@ class BatchScenes:
                 lt.add_light(**l)
             lt.set_ambient_light(w['ambient'])
             d = self.descs[i]
-            d.width, d.height, d.nfaces = W, H, len(w['verts'])
+            d.width, d.height, d.nfaces = W, H, T
             d.raster_flags = (1 if culling else 0) | (2 if clipping else 0)
             d.shade_flags = _abi.VCOLOR | _abi.BG | (_abi.TONEMAP if tonemap else 0) | \
                 (_abi.SMOOTH if norms is not None else 0)
@@ -91,13 +96,13 @@ class BatchScenes:
             d.cam.bias[:] = [0.5, 0.5]
             d.mesh.verts, d.mesh.colors = _abi.ptr(verts), _abi.ptr(colors)
             d.mesh.norms = _abi.ptr(norms) if norms is not None else 0
-            d.mesh.nfaces, d.mesh.id_base = len(w['verts']), 0
+            d.mesh.nfaces, d.mesh.id_base = T, 0
             d.mat = lo.struct
             d.lights = lt.to_struct()
             self._keep += [verts, colors, norms, attrs, lo]
             self.images.append(img)
             self.res.append((W, H))
-            self.nfaces.append(len(w['verts']))
+            self.nfaces.append(T)
 
     @property
     def total_faces(self):

@@ -30,14 +30,18 @@ constexpr int kTileLog = 4;
 constexpr int kTile = 1 << kTileLog;  // 16 x 16 pixel tiles, 2 KB of keys in LDS
 constexpr int kBinBlock = 1024;       // lanes of a binning workgroup
 constexpr int kBinMaxTiles = 64;      // bbox tiles above which a face is rasterised as a giant
-constexpr int kMaxBinGroups = 8192;   // binning workgroups a k_tile workgroup can index
+// binning workgroups a k_tile workgroup can index: its g + 1 words of dynamic
+// LDS plus ~21 KB of static LDS stay within a 64 KB workgroup allocation
+constexpr int kMaxBinGroups = 8192;
+// bytes of dynamic LDS a binning workgroup may take (its static LDS is 72 bytes)
+constexpr int kLdsBudget = 65536 - 128;
 
 // Faces per binning workgroup for a screen of ntiles tiles: the largest of 8192,
-// 4096, 2048 and 1024 whose LDS tables (ntiles + faces words) fit in 64 KB, or 0.
-// Larger groups mean fewer (group, tile) chunk records for k_tile to read.
+// 4096, 2048 and 1024 whose LDS tables (ntiles + faces words) fit the budget,
+// or 0.  Larger groups mean fewer (group, tile) chunk records for k_tile to read.
 inline int bin_faces_for(long ntiles) {
   for (int f = 8192; f >= 1024; f >>= 1)
-    if ((ntiles + f) * 4 <= 65536) return f;
+    if ((ntiles + f) * 4 <= kLdsBudget) return f;
   return 0;
 }
 
@@ -205,7 +209,7 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Tile t of job j (the body of k_tile).  Needs (ngroups + 1) words of dynamic LDS.
+// Tile t of job j (the body of k_tile).  Needs ngroups + 1 words of dynamic LDS.
 __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
   __shared__ unsigned long long tk[kTile * kTile];
@@ -225,6 +229,8 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   const int tx = t % tiles_x, ty = t / tiles_x;
   const int ox = tx << kTileLog, oy = ty << kTileLog;
   const int ntiles = j.ntiles;  // chunk tables are group-major: [group][tile]
+  // (Staging the chunks' entry offsets in LDS with the counts, to take one
+  // dependent read out of the face loop, measured no faster.)
   const uint32_t total = block_exclusive_scan<kBlock>(
       ngroups, [&](int c) { return chunk_cnt[(size_t)c * ntiles + t]; },
       [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);

@@ -641,7 +641,7 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     const int tx = (d.width + kTile - 1) / kTile, ty = (d.height + kTile - 1) / kTile;
     const int ntiles = tx * ty;
     const int ngroups = (d.nfaces + bf - 1) / bf;
-    if ((ntiles + bf) * 4 > 65536) {
+    if ((ntiles + bf) * 4 > kLdsBudget || ngroups > kMaxBinGroups) {
       batch_free(b);
       return fail(nullptr, TINA_ERR_CAPACITY, "scene " + std::to_string(i) + ": screen too large for bin_faces");
     }

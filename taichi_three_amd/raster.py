@@ -96,7 +96,7 @@ class TriangleRaster:
             f |= _abi.SMOOTH
         if self.texturing:
             f |= _abi.TEX
-        if self._fb['colors'] is not None:
+        if self._fb is not None and self._fb['colors'] is not None:
             f |= _abi.VCOLOR
         return f
 
@@ -127,8 +127,9 @@ class TriangleRaster:
         lights = lighting.to_struct()
         img = main.img.storage
         mesh = self.mesh_struct() if self._fb is not None else _abi.Mesh()
-        if self._fb is None:
-            mesh.verts = 1  # never dereferenced: no face id can fall in an empty range
+        if self._fb is None or self.nfaces == 0:
+            mesh.nfaces = 0
+            mesh.verts = mesh.verts or 1  # never dereferenced: no face id can fall in an empty range
         bg = (ctypes.c_float * 3)(*[float(x) for x in np.broadcast_to(np.asarray(bgcolor, dtype=np.float32), 3)])
         if neural is not None and self._fb is not None and lo.is_pbr and neural.lighting is lighting \
                 and (flags & _abi.TONEMAP) and not (flags & (_abi.TEX | _abi.NOLIGHT)):

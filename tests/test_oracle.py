@@ -83,7 +83,8 @@ def raycast_f64(W, H, view, proj, verts, culling=True, clipping=True):
         margin = np.where(near_edge, 0.0, margin)
         ids = np.where(better, f, ids)
         z = np.where(better, zz, z)
-    gap = second - z
+    with np.errstate(invalid='ignore'):  # inf - inf where no face is hit
+        gap = second - z
     margin = np.minimum(margin, np.where(np.isfinite(gap), gap * 1e3, np.inf))
     return ids, z, margin
 
@@ -170,7 +171,9 @@ def test_orthographic_exact_edges(seed):
                 P = np.array([x + 0.5, y + 0.5])
                 V = [np.array([float(A[0]), float(A[1])]), np.array([float(B[0]), float(B[1])]),
                      np.array([float(C[0]), float(C[1])])]
-                dist = min(abs(np.cross(V[(k + 1) % 3] - V[k], P - V[k])) / np.linalg.norm(V[(k + 1) % 3] - V[k])
+                def cross2(u, v):
+                    return u[0] * v[1] - u[1] * v[0]
+                dist = min(abs(cross2(V[(k + 1) % 3] - V[k], P - V[k])) / np.linalg.norm(V[(k + 1) % 3] - V[k])
                            for k in range(3))
                 if dist > 1e-3:
                     assert (occ[y, x] >= 0) == e, (f, x, y)
