@@ -102,6 +102,36 @@ def psnr(a, b):
     return 10.0 * np.log10(1.0 / max(mse, 1e-30))
 
 
+def eval_only(ns, n, reps):
+    """The two evaluations alone, without the per-pixel fetch of normals,
+    material and view that the shade pass shares: the network and the analytic
+    Cook-Torrance + IBL shader on the same n feature rows (drawn like the
+    training pairs, seed 7), device ms per call and the PSNR between them."""
+    import numpy as np
+    import torch
+    X, _ = ns.samples(n, seed=7)
+
+    def timed(fn):
+        fn(X)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn(X)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    t_nn, t_an = timed(ns.infer), timed(ns.analytic)
+    o_nn = ns.infer(X)[:, :3].cpu().numpy()
+    o_an = ns.analytic(X)[:, :3].cpu().numpy()
+    return {'rows': n, 'analytic_ms': round(t_an, 4), 'mlp_ms': round(t_nn, 4),
+            'speedup': round(t_an / t_nn, 3), 'psnr_db': round(psnr(o_nn, o_an), 2),
+            'mlp_tflops': round(n * 2 * (12 * 64 + 64 * 64 * 2 + 64 * 4) / (t_nn * 1e-3) / 1e12, 2)}
+
+
 def neural_vs_analytic(reps, train_steps):
     """The learned surrogate against the analytic Cook-Torrance + IBL shade at
     1920x1080 with 1M faces: train on 16M analytic pairs, then time the shade
@@ -128,9 +158,11 @@ def neural_vs_analytic(reps, train_steps):
     neural = stage_times(scene, reps)['shade']
     ids = scene.engine.ids.to_numpy()
     covered = int((ids >= 0).sum())
+    rows = eval_only(ns, covered, reps)
     return {'workload': 'ibl_1080p: 1920x1080, 1M-face soup, PBR under a 512x256 procedural HDR sky + 1 sun',
             'analytic_shade_ms': round(analytic, 4), 'neural_shade_ms': round(neural, 4),
             'speedup': round(analytic / neural, 3), 'psnr_db': round(psnr(img, ref), 2),
+            'eval_only': rows,
             'fallback_pixels': nfall, 'covered_pixels': covered,
             'fallback_frac': round(nfall / max(1, covered), 4), 'threshold': ns.threshold,
             'train': {'samples': 1 << 24, 'steps': train_steps, 'batch': 16384, 'seconds': round(t_train, 2)},

@@ -239,10 +239,17 @@ int tina_mlp_train_step(const float *d_x, const float *d_y, int64_t first, int64
                         float *d_params, float *d_m1, float *d_m2, float *d_partial, float lr, int step,
                         float *d_grad, void *stream);
 
+/* The analytic PBR + IBL + light-0 shader (ACES) on feature rows d_x [n][12] ->
+ * d_out [n][4]: the evaluation the network replaces, on the network's inputs
+ * (V is recovered as 2 (n.R) n - R, roughness as exp(x7)). */
+int tina_nn_analytic_rows(const tina_lights *lights, const tina_ibl *ibl, const float *d_x, int64_t n,
+                          float specular, float *d_out, void *stream);
+
 /* float32 parameters -> the packed bf16 inference weights (TINA_NN_PACK_BYTES). */
 int tina_mlp_pack(const float *d_params, void *d_packed, void *stream);
 
-/* The packed network on d_x [n][12] -> d_out [n][4] (max(0, colour), error). */
+/* The packed network on d_x [n][12] -> d_out [n][4] (max(0, colour), error);
+ * both buffers 16-byte aligned. */
 int tina_mlp_inference(const void *d_packed, const float *d_x, int64_t n, float *d_out, void *stream);
 
 /* tina_shade for the PBR material under IBL (TINA_IBL | TINA_TONEMAP required)

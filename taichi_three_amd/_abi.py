@@ -133,6 +133,8 @@ EXPORTS = [
     ('tina_mlp_train_step', _i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp, ctypes.c_float, _i32, _vp,
                                    _vp]),
     ('tina_mlp_pack', _i32, [_vp, _vp, _vp]),
+    ('tina_nn_analytic_rows', _i32, [ctypes.POINTER(Lights), ctypes.POINTER(IBLTables), _vp, _i64,
+                                     ctypes.c_float, _vp, _vp]),
     ('tina_mlp_inference', _i32, [_vp, _vp, _i64, _vp, _vp]),
     ('tina_shade_nn', _i32, [_vp, ctypes.POINTER(Mesh), ctypes.POINTER(Material), ctypes.POINTER(Lights),
                              ctypes.POINTER(IBLTables), _vp, _vp, ctypes.POINTER(ctypes.c_float * 3), _u32,

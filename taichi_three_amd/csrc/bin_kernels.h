@@ -27,6 +27,8 @@
 namespace tina {
 
 constexpr int kTileLog = 4;
+// a queued fragment packs its face slot in the wave (6 bits) and its tile pixel in 16 bits
+static_assert(6 + 2 * kTileLog <= 16, "fragment code does not fit 16 bits");
 constexpr int kTile = 1 << kTileLog;  // 16 x 16 pixel tiles, 2 KB of keys in LDS
 constexpr int kBinBlock = 1024;       // lanes of a binning workgroup
 constexpr int kBinMaxTiles = 64;      // bbox tiles above which a face is rasterised as a giant
@@ -129,6 +131,8 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   for (int t = tid; t < ntiles; t += kBinBlock) cnt[t] = 0;
   __syncthreads();
 
+  // (Loading four faces per lane before setting any up measured no faster: the
+  // extra registers cost more occupancy than the overlapped loads gained.)
   for (int i = tid; i < nbf; i += kBinBlock) {
     const long f = f0 + i;
     uint32_t sp = 0xFFFFFFFFu;
@@ -289,7 +293,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     int qn = 0;  // uniform: pending fragments in the queue
     auto exact = [&](int e) {  // exact test + depth + LDS min for queue entry e
       const int code = q[e];
-      const int j = wbase + (code >> 8), lx = code & 0xF, ly = (code >> 4) & 0xF;
+      const int j = wbase + (code >> (2 * kTileLog)), lx = code & (kTile - 1), ly = (code >> kTileLog) & (kTile - 1);
       const float4 r0 = R0[j], r1 = R1[j], r2 = R2[j], r3 = R3[j];
       FaceSetup s;
       s.bx = r0.x; s.by = r0.y; s.cx = r0.z; s.cy = r0.w;
@@ -316,7 +320,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
         const int packed = __float_as_int(R3[j].z);
         const int lx0 = packed & 0xFF, ly0 = (packed >> 8) & 0xFF, wj = (packed >> 16) & 0xFF;
         const int li = gI - start;
-        // li / wj: (li + 1/2) / wj is >= 1/32 from an integer (li < 256, wj <= 16),
+        // li / wj: (li + 1/2) / wj is >= 1/(2 kTile) from an integer (li < kTile^2, wj <= kTile),
         // far more than the error of a multiply by the hardware reciprocal
         const int dy = (int)(((float)li + 0.5f) * __builtin_amdgcn_rcpf((float)wj));
         const int dx = li - dy * wj;
@@ -327,7 +331,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
         s.bcnx = r1.x; s.bcny = r1.y; s.canx = r1.z; s.cany = r1.w;
         keep = !((packed >> 24) &&
                  pixel_rejects(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1]));
-        code = (k << 8) | (ly << 4) | lx;
+        code = (k << (2 * kTileLog)) | (ly << kTileLog) | lx;
       }
       const unsigned long long m = __ballot(keep);
       if (keep) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)code;

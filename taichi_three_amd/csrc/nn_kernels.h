@@ -117,6 +117,34 @@ k_nn_samples(const ShadeArgs a_by_value, uint32_t seed, long nsamples, float r_l
   Y[i * 4 + 3] = 0.0f;
 }
 
+// The analytic shader on feature rows: the Cook-Torrance + IBL evaluation the
+// network replaces, on the same inputs (n, V = 2 (n.R) n - R, roughness =
+// exp(x7), metallic, albedo), for comparing the two evaluations alone.
+__global__ void __launch_bounds__(256)
+k_nn_analytic_rows(const ShadeArgs a_by_value, const float* __restrict__ X, long n, float spec,
+                   float* __restrict__ O) {
+  (void)a_by_value;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const ShadeArgs& a = a_by_value;
+#endif
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* x = X + i * kNnIn;
+  const V3 nrm = v3(x[0], x[1], x[2]), R = v3(x[3], x[4], x[5]);
+  const float t = 2.0f * vdot(nrm, R);
+  const V3 V = v3(t * nrm.x - R.x, t * nrm.y - R.y, t * nrm.z - R.z);
+  const float r = expf(x[7]), m = x[8];
+  const V3 alb = v3(x[9], x[10], x[11]);
+  const V3 f0 = pbr_f0(m, alb, spec);
+  const V3 c = aces(shade_core<PROG_PBR, true>(a, nrm, vs(0.0f), V, alb, f0, f0, r));
+  O[i * 4 + 0] = c.x;
+  O[i * 4 + 1] = c.y;
+  O[i * 4 + 2] = c.z;
+  O[i * 4 + 3] = 0.0f;
+}
+
 // ---- training ------------------------------------------------------------------
 
 // One wave per workgroup, one sample per lane.  Every lane reads a weight at the
@@ -334,103 +362,36 @@ __global__ void __launch_bounds__(256) k_mlp_pack(const float* __restrict__ para
   }
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// bf16(a), bf16(b) (round to nearest even) packed in one word: v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
+// relu(bf16(a)), relu(bf16(b)) = bf16(relu(a)), bf16(relu(b)): rounding keeps the
+// sign, so the clamp can follow it, on the bit patterns -- a bf16 with its sign
+// bit set is a negative int16, and max with 0 makes it +0 (one v_pk_max_i16 for
+// two values, with none of the NaN quieting that fmaxf costs on floats).
+__device__ __forceinline__ uint32_t relu_bf16_pair(float a, float b) {
+  const i16x2 v = __builtin_bit_cast(i16x2, bf16_pair(a, b));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, (i16x2){0, 0}));
+}
+
 // relu + bf16 of the D fragments of row blocks 2kb, 2kb+1: the B operand of K block kb
 __device__ __forceinline__ bf16x8 nn_next_b(const f32x4& lo, const f32x4& hi) {
-  bf16x8 r;
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    r[q] = (__bf16)fmaxf(lo[q], 0.0f);
-    r[4 + q] = (__bf16)fmaxf(hi[q], 0.0f);
-  }
-  return r;
+  const u32x4 r = {relu_bf16_pair(lo[0], lo[1]), relu_bf16_pair(lo[2], lo[3]), relu_bf16_pair(hi[0], hi[1]),
+                   relu_bf16_pair(hi[2], hi[3])};
+  return __builtin_bit_cast(bf16x8, r);
 }
 
+// This lane's A-operand fragments of the four layers (the biases live in LDS).
 struct NnWeights {
   bf16x8 w1[4], w2[8], w3[8], w4[2];
-  f32x4 b1[4], b2[4], b3[4], b4;
 };
-
-__device__ __forceinline__ void nn_load_weights(const __bf16* __restrict__ pk, int lane, NnWeights& w) {
-  const bf16x8* fr = (const bf16x8*)pk;
-  const int g = lane / 16;
-#pragma unroll
-  for (int i = 0; i < 4; i++) w.w1[i] = fr[i * 64 + lane];
-#pragma unroll
-  for (int i = 0; i < 8; i++) w.w2[i] = fr[(4 + i) * 64 + lane];
-#pragma unroll
-  for (int i = 0; i < 8; i++) w.w3[i] = fr[(12 + i) * 64 + lane];
-#pragma unroll
-  for (int i = 0; i < 2; i++) w.w4[i] = fr[(20 + i) * 64 + lane];
-  const float* bias = (const float*)(pk + kNnPackBias);
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      w.b1[mb][q] = bias[16 * mb + 4 * g + q];
-      w.b2[mb][q] = bias[64 + 16 * mb + 4 * g + q];
-      w.b3[mb][q] = bias[128 + 16 * mb + 4 * g + q];
-    }
-#pragma unroll
-  for (int q = 0; q < 4; q++) w.b4[q] = (4 * g + q) < kNnOut ? bias[192 + 4 * g + q] : 0.0f;
-}
-
-// The network on 16 pixels: x = B operand of layer 1 (this lane's 8 features of
-// pixel j); returns D of layer 4 (lanes of group 0 hold outputs 0-3 of pixel j).
-__device__ __forceinline__ f32x4 nn_eval16(const NnWeights& w, bf16x8 x) {
-  f32x4 h[4];
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++) h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w1[mb], x, w.b1[mb], 0, 0, 0);
-  bf16x8 k0 = nn_next_b(h[0], h[1]), k1 = nn_next_b(h[2], h[3]);
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++) {
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb], k0, w.b2[mb], 0, 0, 0);
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb + 1], k1, h[mb], 0, 0, 0);
-  }
-  k0 = nn_next_b(h[0], h[1]);
-  k1 = nn_next_b(h[2], h[3]);
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++) {
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb], k0, w.b3[mb], 0, 0, 0);
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb + 1], k1, h[mb], 0, 0, 0);
-  }
-  k0 = nn_next_b(h[0], h[1]);
-  k1 = nn_next_b(h[2], h[3]);
-  f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[0], k0, w.b4, 0, 0, 0);
-  o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[1], k1, o, 0, 0, 0);
-  return o;
-}
-
-// Test entry: the network on n rows of features X[n][12] -> O[n][4], same
-// arithmetic as k_shade_nn.  One wave per 64 rows.
-__global__ void __launch_bounds__(64)
-k_mlp_infer(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, float* __restrict__ O) {
-  __shared__ __attribute__((aligned(16))) __bf16 sx[64][16];
-  __shared__ float so[64][4];
-  const int lane = threadIdx.x, g = lane / 16, j = lane % 16;
-  NnWeights w;
-  nn_load_weights(pk, lane, w);
-  const long base = (long)blockIdx.x * 64;
-  const long r = base + lane;
-#pragma unroll
-  for (int k = 0; k < 16; k++) sx[lane][k] = (__bf16)(r < n && k < kNnIn ? X[r * kNnIn + k] : 0.0f);
-  __syncthreads();
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++) {
-    bf16x8 x;
-#pragma unroll
-    for (int e = 0; e < 8; e++) x[e] = g < 2 ? sx[16 * nb + j][8 * g + e] : (__bf16)0.0f;
-    const f32x4 o = nn_eval16(w, x);
-    if (g == 0) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) so[16 * nb + j][q] = o[q];
-    }
-  }
-  __syncthreads();
-  if (r < n) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) O[r * 4 + q] = q < 3 ? fmaxf(so[lane][q], 0.0f) : so[lane][q];
-  }
-}
 
 // The analytic shader over the queued pixels fbq[0 .. *fbn); also clears
 // *fbn_next, the counter the next k_shade_nn launch appends to.

@@ -73,33 +73,6 @@ __device__ __forceinline__ void nn_inputs(const ShadeArgs& a, uint32_t f, int px
   V = vsmul(-__builtin_amdgcn_rsqf(vdot(d, d)), d);
 }
 
-// The network on one 16-pixel block with the biases read from LDS into the
-// accumulators.
-__device__ __forceinline__ f32x4 nn_eval16_lds(const NnWeights& w, const float* __restrict__ sb, int g, bf16x8 x) {
-  auto bias = [&](int base, int mb) { return *(const f32x4*)(sb + base + 16 * mb + 4 * g); };
-  f32x4 h[4];
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++) h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w1[mb], x, bias(0, mb), 0, 0, 0);
-  bf16x8 k0 = nn_next_b(h[0], h[1]), k1 = nn_next_b(h[2], h[3]);
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++) {
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb], k0, bias(64, mb), 0, 0, 0);
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb + 1], k1, h[mb], 0, 0, 0);
-  }
-  k0 = nn_next_b(h[0], h[1]);
-  k1 = nn_next_b(h[2], h[3]);
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++) {
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb], k0, bias(128, mb), 0, 0, 0);
-    h[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb + 1], k1, h[mb], 0, 0, 0);
-  }
-  k0 = nn_next_b(h[0], h[1]);
-  k1 = nn_next_b(h[2], h[3]);
-  f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[0], k0, bias(192, 0), 0, 0, 0);
-  o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[1], k1, o, 0, 0, 0);
-  return o;
-}
-
 // The network on four 16-pixel blocks at once, layer by layer, so that every
 // layer issues 16 independent MFMAs (4 row blocks x 4 pixel blocks) back to
 // back instead of one dependent chain per block.
@@ -156,6 +129,74 @@ __device__ __forceinline__ void nn_eval64(const NnWeights& w, const float* __res
 
 constexpr int kNnShadeGrid = 1024;  // persistent workgroups of k_shade_nn
 
+// Loads the packed weight fragments into this lane's registers and pins them
+// there (otherwise the compiler re-reads them inside the pixel loop), and the
+// biases into LDS (b1, b2, b3, then b4 padded to 16 rows).
+__device__ __forceinline__ void nn_resident(const __bf16* __restrict__ pk, int lane, NnWeights& w, float* s_b) {
+  const float* bias = (const float*)(pk + kNnPackBias);
+  for (int i = threadIdx.x; i < 3 * 64 + 16; i += blockDim.x) s_b[i] = i < 3 * 64 + kNnOut ? bias[i] : 0.0f;
+  const bf16x8* fr = (const bf16x8*)pk;
+#pragma unroll
+  for (int i = 0; i < 4; i++) w.w1[i] = fr[i * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w.w2[i] = fr[(4 + i) * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 8; i++) w.w3[i] = fr[(12 + i) * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 2; i++) w.w4[i] = fr[(20 + i) * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 4; i++) asm volatile("" : "+v"(w.w1[i]));
+#pragma unroll
+  for (int i = 0; i < 8; i++) asm volatile("" : "+v"(w.w2[i]));
+#pragma unroll
+  for (int i = 0; i < 8; i++) asm volatile("" : "+v"(w.w3[i]));
+#pragma unroll
+  for (int i = 0; i < 2; i++) asm volatile("" : "+v"(w.w4[i]));
+}
+
+// The network on n feature rows X[n][12] -> O[n][4] (max(0, colour), error
+// estimate), with k_shade_nn's arithmetic: persistent workgroups, each wave
+// taking 64 rows at a time.  The B operand of layer 1 is read straight from the
+// rows (lane (g, j) of block nb holds features 8g..8g+7 of row 16 nb + j: two
+// 16-byte loads for g = 0, one for g = 1, zeros above feature 11) and the lanes
+// of group 0 store the outputs of their rows, so nothing passes through LDS
+// but the biases.
+__global__ void __launch_bounds__(kBlock)
+k_mlp_rows(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, float* __restrict__ O) {
+  __shared__ __attribute__((aligned(16))) float s_b[3 * 64 + 16];
+  const int lane = __lane_id(), wv = threadIdx.x / 64, g = lane / 16, j = lane % 16;
+  NnWeights w;
+  nn_resident(pk, lane, w, s_b);
+  __syncthreads();
+  const long nchunks = (n + 63) / 64;
+  for (long c = (long)blockIdx.x * (kBlock / 64) + wv; c < nchunks; c += (long)gridDim.x * (kBlock / 64)) {
+    bf16x8 xb[4];
+    f32x4 ob[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const long r = c * 64 + 16 * nb + j;
+      const float4* row = (const float4*)(X + r * kNnIn);
+      float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q = p;
+      if (r < n && g < 2) {
+        p = row[2 * g];
+        if (g == 0) q = row[1];
+      }
+      const u32x4 v = {bf16_pair(p.x, p.y), bf16_pair(p.z, p.w), bf16_pair(q.x, q.y), bf16_pair(q.z, q.w)};
+      xb[nb] = __builtin_bit_cast(bf16x8, v);
+    }
+    nn_eval64(w, s_b, g, xb, ob);
+    if (g == 0) {
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) {
+        const long r = c * 64 + 16 * nb + j;
+        if (r < n)
+          *(float4*)(O + r * 4) = make_float4(fmaxf(ob[nb][0], 0.0f), fmaxf(ob[nb][1], 0.0f), fmaxf(ob[nb][2], 0.0f),
+                                              ob[nb][3]);
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const __bf16* __restrict__ pk,
            float threshold, uint32_t* __restrict__ fbq, uint32_t* __restrict__ fbn, float* __restrict__ img) {
@@ -177,32 +218,8 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
   const long npix = (long)a.cam.W * a.cam.H;
   const long nchunks = (npix + kShadePix - 1) / kShadePix;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  {
-    const float* bias = (const float*)(pk + kNnPackBias);
-    for (int i = tid; i < 3 * 64 + 16; i += kBlock) s_b[i] = i < 3 * 64 + kNnOut ? bias[i] : 0.0f;
-  }
   NnWeights w;
-  {
-    const bf16x8* fr = (const bf16x8*)pk;
-#pragma unroll
-    for (int i = 0; i < 4; i++) w.w1[i] = fr[i * 64 + lane];
-#pragma unroll
-    for (int i = 0; i < 8; i++) w.w2[i] = fr[(4 + i) * 64 + lane];
-#pragma unroll
-    for (int i = 0; i < 8; i++) w.w3[i] = fr[(12 + i) * 64 + lane];
-#pragma unroll
-    for (int i = 0; i < 2; i++) w.w4[i] = fr[(20 + i) * 64 + lane];
-    // Keep the fragments resident: without this the compiler re-reads them from
-    // memory inside the pixel loop (a 22 KB reload per 64 pixels).
-#pragma unroll
-    for (int i = 0; i < 4; i++) asm volatile("" : "+v"(w.w1[i]));
-#pragma unroll
-    for (int i = 0; i < 8; i++) asm volatile("" : "+v"(w.w2[i]));
-#pragma unroll
-    for (int i = 0; i < 8; i++) asm volatile("" : "+v"(w.w3[i]));
-#pragma unroll
-    for (int i = 0; i < 2; i++) asm volatile("" : "+v"(w.w4[i]));
-  }
+  nn_resident(pk, lane, w, s_b);
   const tina_material& m = a.mat;
   const int fres = m.ops[2].b;  // FresnelFactor slot: metallic = its first operand
   const V3 L = light_at(a.lights, 0).xyz;
@@ -267,9 +284,9 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
         x[8] = metal;
         x[9] = albedo.x; x[10] = albedo.y; x[11] = albedo.z;
       }
-      __bf16* row = s_x[wv][lane];
-#pragma unroll
-      for (int k = 0; k < 16; k++) row[k] = (__bf16)(k < kNnIn ? x[k] : 0.0f);
+      u32x4* row = (u32x4*)s_x[wv][lane];
+      row[0] = (u32x4){bf16_pair(x[0], x[1]), bf16_pair(x[2], x[3]), bf16_pair(x[4], x[5]), bf16_pair(x[6], x[7])};
+      row[1] = (u32x4){bf16_pair(x[8], x[9]), bf16_pair(x[10], x[11]), 0u, 0u};
       wave_sync_lds();
       bf16x8 xb[4];
       f32x4 ob[4];

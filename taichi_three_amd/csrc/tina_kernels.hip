@@ -430,6 +430,24 @@ int tina_nn_samples(const tina_lights* lights, const tina_ibl* ibl, uint32_t see
   return hip_check(nullptr, hipGetLastError(), "k_nn_samples");
 }
 
+int tina_nn_analytic_rows(const tina_lights* lights, const tina_ibl* ibl, const float* d_x, int64_t n, float specular,
+                          float* d_out, void* stream) {
+  int rc = check_sun(nullptr, lights);
+  if (rc) return rc;
+  if ((rc = check_ibl(nullptr, ibl))) return rc;
+  if (n < 0) return fail(nullptr, TINA_ERR_INVALID, "tina_nn_analytic_rows: n < 0");
+  if (n == 0) return TINA_OK;
+  if (!d_x || !d_out) return fail(nullptr, TINA_ERR_INVALID, "tina_nn_analytic_rows: buffer is NULL");
+  ShadeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.lights = *lights;
+  a.ibl = *ibl;
+  a.flags = TINA_IBL;
+  hipLaunchKernelGGL(k_nn_analytic_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
+                     d_x, (long)n, specular, d_out);
+  return hip_check(nullptr, hipGetLastError(), "k_nn_analytic_rows");
+}
+
 int tina_mlp_train_step(const float* d_x, const float* d_y, int64_t first, int64_t nsamples, int batch,
                         float* d_params, float* d_m1, float* d_m2, float* d_partial, float lr, int step,
                         float* d_grad, void* stream) {
@@ -459,12 +477,15 @@ int tina_mlp_pack(const float* d_params, void* d_packed, void* stream) {
 }
 
 int tina_mlp_inference(const void* d_packed, const float* d_x, int64_t n, float* d_out, void* stream) {
-  if (!d_packed || !d_x || !d_out) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: buffer is NULL");
   if (n < 0) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: n < 0");
+  if (!d_packed) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: network is NULL");
   if (n == 0) return TINA_OK;
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
+  if (!d_x || !d_out) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: buffer is NULL");
+  if (((uintptr_t)d_x | (uintptr_t)d_out) & 15)
+    return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: rows must be 16-byte aligned");
+  hipLaunchKernelGGL(k_mlp_rows, dim3(grid_for(n, kBlock, kNnShadeGrid)), dim3(kBlock), 0, (hipStream_t)stream,
                      (const __bf16*)d_packed, d_x, (long)n, d_out);
-  return hip_check(nullptr, hipGetLastError(), "k_mlp_infer");
+  return hip_check(nullptr, hipGetLastError(), "k_mlp_rows");
 }
 
 int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,

@@ -150,6 +150,16 @@ class NeuralShading:
                                                   _abi.ptr(O), _abi.stream_handle()))
         return O
 
+    def analytic(self, X):
+        """The analytic shader on the same feature rows: [n][4] (ACES colour, 0)."""
+        torch = _torch()
+        O = torch.empty((X.shape[0], 4), dtype=torch.float32, device=self.device)
+        lt = self.lighting.to_struct()
+        _abi.check(_abi.load().tina_nn_analytic_rows(
+            ctypes.byref(lt), ctypes.byref(self.lighting.ibltab.struct), _abi.ptr(X), int(X.shape[0]),
+            ctypes.c_float(self.specular), _abi.ptr(O), _abi.stream_handle()))
+        return O
+
     def fallback_count(self, engine):
         """Pixels the last neural shade of `engine` handed to the analytic kernel."""
         torch = _torch()

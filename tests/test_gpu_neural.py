@@ -83,8 +83,9 @@ def test_adam_step_matches_the_numpy_oracle(gpu):
 
 
 def test_mfma_network_matches_the_bf16_oracle(gpu):
-    """k_mlp_pack + k_mlp_infer (the MFMA layout and the permuted K order) vs
-    the oracle's bf16 arithmetic, for random weights and biases."""
+    """k_mlp_pack + k_mlp_rows (the MFMA layout and the permuted K order) vs
+    the oracle's bf16 arithmetic, for random weights and biases: 1000 rows (a
+    ragged last wave) and 300,000 rows (every persistent wave loops)."""
     import torch
     from taichi_three_amd.neural import NeuralShading
     print('case: nn mfma vs oracle', flush=True)
@@ -93,12 +94,30 @@ def test_mfma_network_matches_the_bf16_oracle(gpu):
     p = nn.init_params(5) + 0.05 * rng.normal(size=nn.NPARAMS)
     ns.params.copy_(torch.from_numpy(p.astype(np.float32)))
     ns.pack()
-    X, _ = ns.samples(1000, seed=13)
-    O = ns.infer(X).cpu().numpy()
-    ref = nn.infer_bf16(p.astype(np.float32), X.cpu().numpy())
-    scale = np.abs(ref).max()
-    assert np.abs(O - ref).max() <= 2e-2 * scale, np.abs(O - ref).max() / scale
-    assert np.abs(O - ref).mean() <= 2e-3 * scale
+    for n in (1000, 300_000):
+        X, _ = ns.samples(n, seed=13)
+        O = ns.infer(X).cpu().numpy()
+        ref = nn.infer_bf16(p.astype(np.float32), X.cpu().numpy())
+        scale = np.abs(ref).max()
+        assert np.abs(O - ref).max() <= 2e-2 * scale, (n, np.abs(O - ref).max() / scale)
+        assert np.abs(O - ref).mean() <= 2e-3 * scale, n
+
+
+def test_analytic_rows_reproduce_the_sample_targets(gpu):
+    """tina_nn_analytic_rows recovers V and the roughness from the features and
+    shades them: on rough surfaces it reproduces k_nn_samples' targets, which the
+    oracle pins above; an empty call writes nothing."""
+    import torch
+    from taichi_three_amd.neural import NeuralShading
+    print('case: nn analytic rows vs samples', flush=True)
+    ns = NeuralShading(_lighting(), seed=6)
+    X, Y = ns.samples(100_003, seed=17, roughness=(0.3, 1.0), p_metal=-1.0)
+    O = ns.analytic(X)
+    err = (O[:, :3] - Y[:, :3]).abs()
+    assert float(err.max()) <= 1e-4, float(err.max())
+    assert bool((O[:, 3] == 0).all())
+    assert ns.analytic(X[:0]).shape == (0, 4)
+    assert ns.infer(X[:0]).shape == (0, 4)
 
 
 def _psnr(a, b):
