@@ -218,7 +218,8 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
   __shared__ unsigned long long tk[kTile * kTile];
   __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
-  __shared__ int32_t Lend[kBlock];
+  __shared__ int32_t Lstart[kBlock];  // per wave: first fragment of each live face
+  __shared__ int32_t Mark[kBlock];    // per wave: window tag at the position of a face's last fragment
   __shared__ uint16_t Q[kBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
   __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
@@ -257,6 +258,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     if (b0 >= total) break;
     const uint32_t v = b0 + lane;
     int area = 0;
+    float4 ra, rb, rc, rd;
     if (v < total) {
       int lo = 0, hi = ngroups;  // pre[lo] <= v < pre[hi]
       while (hi - lo > 1) {
@@ -272,25 +274,41 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
       if (w > 0 && h > 0) {
         area = w * h;
-        R0[tid] = r.a;
-        R1[tid] = r.b;
-        R2[tid] = r.c;
-        // Bz, Cz, clipped origin (x | y << 16), width | face id ... packed below
-        R3[tid] = make_float4(r.d.x, r.d.y,
-                              __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) | (wellc << 24)),
-                              __int_as_float((int)(id_base + fid)));
+        ra = r.a;
+        rb = r.b;
+        rc = r.c;
+        // Bz, Cz, clipped origin (x | y << 8), width, well-conditioned bit; face id
+        rd = make_float4(r.d.x, r.d.y,
+                         __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) | (wellc << 24)),
+                         __int_as_float((int)(id_base + fid)));
       }
     }
+    // The faces with pixels in the tile take consecutive slots, so their
+    // fragment ranges [start, end) tile [0, wtotal) with no empty range.
+    const unsigned long long live = __ballot(area > 0);
     const int end = wave_inclusive_scan(area, lane);
-    Lend[tid] = end;
     const int wtotal = __shfl(end, 63, 64);
+    if (area > 0) {
+      const int slot = wbase + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)live, 0u));
+      R0[slot] = ra;
+      R1[slot] = rb;
+      R2[slot] = rc;
+      R3[slot] = rd;
+      Lstart[slot] = end - area;
+    }
+    Mark[tid] = -1;
     wave_sync_lds();
-    // Cheap pass over every fragment: find its face (binary search over the wave's
-    // prefix ends), evaluate the unscaled edge values, and drop it when a
+    // Cheap pass over every fragment.  Fragment g belongs to the first face whose
+    // last fragment is >= g: per window of 64 fragments, the faces whose last
+    // fragment falls inside it tag that position, and a ballot of the tags gives
+    // each lane its face as (faces ended before the window) + (tags below it).
+    // Each fragment then evaluates the unscaled edge values and is dropped when a
     // well-conditioned face rejects it.  Survivors (about a fifth) are compacted
     // into the wave's LDS queue and the exact test runs on full waves of them.
     uint16_t* const q = Q + 2 * wbase;
-    int qn = 0;  // uniform: pending fragments in the queue
+    int qn = 0;     // uniform: pending fragments in the queue
+    int kbase = 0;  // uniform: faces whose fragments all precede the window
     auto exact = [&](int e) {  // exact test + depth + LDS min for queue entry e
       const int code = q[e];
       const int j = wbase + (code >> (2 * kTileLog)), lx = code & (kTile - 1), ly = (code >> kTileLog) & (kTile - 1);
@@ -307,16 +325,17 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       }
     };
     for (int g0 = 0; g0 < wtotal; g0 += 64) {
+      if (area > 0 && end - 1 >= g0 && end - 1 < g0 + 64) Mark[wbase + end - 1 - g0] = g0;
+      wave_sync_lds();
+      const unsigned long long ends = __ballot(Mark[tid] == g0);
       const int gI = g0 + lane;
       bool keep = false;
       int code = 0;
       if (gI < wtotal) {
-        int k = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (Lend[wbase + k + step - 1] <= gI) k += step;
+        const int k = kbase + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ends >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)ends, 0u));
         const int j = wbase + k;
-        const int start = k > 0 ? Lend[j - 1] : 0;
+        const int start = Lstart[j];
         const int packed = __float_as_int(R3[j].z);
         const int lx0 = packed & 0xFF, ly0 = (packed >> 8) & 0xFF, wj = (packed >> 16) & 0xFF;
         const int li = gI - start;
@@ -333,6 +352,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
                  pixel_rejects(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1]));
         code = (k << (2 * kTileLog)) | (ly << kTileLog) | lx;
       }
+      kbase += __popcll(ends);
       const unsigned long long m = __ballot(keep);
       if (keep) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)code;
       qn += __popcll(m);
