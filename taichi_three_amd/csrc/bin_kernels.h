@@ -29,6 +29,8 @@ namespace tina {
 constexpr int kTileLog = 4;
 // a queued fragment packs its face slot in the wave (6 bits) and its tile pixel in 16 bits
 static_assert(6 + 2 * kTileLog <= 16, "fragment code does not fit 16 bits");
+// a wave's fragment offsets (64 faces of at most kTile^2 pixels) fit 14 bits
+static_assert(64 << (2 * kTileLog) <= (1 << 14), "fragment offset does not fit 14 bits");
 constexpr int kTile = 1 << kTileLog;  // 16 x 16 pixel tiles, 2 KB of keys in LDS
 constexpr int kBinBlock = 1024;       // lanes of a binning workgroup
 constexpr int kBinMaxTiles = 64;      // bbox tiles above which a face is rasterised as a giant
@@ -131,14 +133,34 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   for (int t = tid; t < ntiles; t += kBinBlock) cnt[t] = 0;
   __syncthreads();
 
-  // (Loading four faces per lane before setting any up measured no faster: the
-  // extra registers cost more occupancy than the overlapped loads gained.)
+  // The positions of a lane's faces are fetched two faces ahead of their set-up,
+  // so that the loads overlap the arithmetic of the faces before them.  (A
+  // binning grid has few workgroups -- one per bin_faces faces -- so occupancy
+  // is set by the grid, not by these registers.)
+  const float* __restrict__ verts = j.verts;
+  auto fetch = [&](int i, float (&v)[9]) {
+    const long f = f0 + i;
+    if (i < nbf && f < j.T) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) v[k] = verts[f * 9 + k];
+    }
+  };
+  float p0[9], p1[9];
+  fetch(tid, p0);
+  fetch(tid + kBinBlock, p1);
   for (int i = tid; i < nbf; i += kBinBlock) {
     const long f = f0 + i;
+    float v[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      v[k] = p0[k];
+      p0[k] = p1[k];
+    }
+    fetch(i + 2 * kBinBlock, p1);
     uint32_t sp = 0xFFFFFFFFu;
     bool giant = false;
     FaceSetup s;
-    if (f < j.T && setup_face(cam, j.verts + f * 9, j.flags, s) && s.topx >= s.botx && s.topy >= s.boty) {
+    if (f < j.T && setup_face(cam, v, j.flags, s) && s.topx >= s.botx && s.topy >= s.boty) {
       const int tx0 = s.botx >> kTileLog, tx1 = s.topx >> kTileLog;
       const int ty0 = s.boty >> kTileLog, ty1 = s.topy >> kTileLog;
       const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1;
@@ -218,7 +240,8 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
   __shared__ unsigned long long tk[kTile * kTile];
   __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
-  __shared__ int32_t Lstart[kBlock];  // per wave: first fragment of each live face
+  // per wave, per live face: its first fragment (bits 0-13) and ceil(2**16 / width) (bits 14-30)
+  __shared__ uint32_t Lstart[kBlock];
   __shared__ int32_t Mark[kBlock];    // per wave: window tag at the position of a face's last fragment
   __shared__ uint16_t Q[kBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
   __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
@@ -295,7 +318,8 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       R1[slot] = rb;
       R2[slot] = rc;
       R3[slot] = rd;
-      Lstart[slot] = end - area;
+      const uint32_t wd = (uint32_t)((__float_as_int(rd.z) >> 16) & 0xFF);
+      Lstart[slot] = (uint32_t)(end - area) | (((65536u + wd - 1u) / wd) << 14);
     }
     Mark[tid] = -1;
     wave_sync_lds();
@@ -335,14 +359,15 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
         const int k = kbase + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ends >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)ends, 0u));
         const int j = wbase + k;
-        const int start = Lstart[j];
+        const uint32_t ls = Lstart[j];
         const int packed = __float_as_int(R3[j].z);
         const int lx0 = packed & 0xFF, ly0 = (packed >> 8) & 0xFF, wj = (packed >> 16) & 0xFF;
-        const int li = gI - start;
-        // li / wj: (li + 1/2) / wj is >= 1/(2 kTile) from an integer (li < kTile^2, wj <= kTile),
-        // far more than the error of a multiply by the hardware reciprocal
-        const int dy = (int)(((float)li + 0.5f) * __builtin_amdgcn_rcpf((float)wj));
-        const int dx = li - dy * wj;
+        const int li = gI - (int)(ls & 0x3FFFu);
+        // li / wj as (li * ceil(2**16 / wj)) >> 16: the multiplier exceeds 2**16 / wj by
+        // less than 1, so the quotient gains less than li / 2**16 < 1/256, while li / wj
+        // sits at least 1/wj >= 1/kTile below the next integer (li < kTile^2, wj <= kTile)
+        const int dy = (int)(__umul24((uint32_t)li, ls >> 14) >> 16);
+        const int dx = li - (int)__umul24((uint32_t)dy, (uint32_t)wj);
         const int lx = lx0 + dx, ly = ly0 + dy;
         const float4 r0 = R0[j], r1 = R1[j];
         FaceSetup s;
@@ -354,7 +379,8 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       }
       kbase += __popcll(ends);
       const unsigned long long m = __ballot(keep);
-      if (keep) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)code;
+      if (keep) q[qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+          (uint16_t)code;
       qn += __popcll(m);
       wave_sync_lds();
       if (qn >= 64) {  // a full wave of survivors
