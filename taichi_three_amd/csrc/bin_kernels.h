@@ -115,6 +115,67 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(int n, Get v, Put out, 
   return *carry;
 }
 
+// Shrinks the pixel bbox of a well-conditioned face to the pixels that can
+// survive the cheap rejection (and hence the coverage test); false when none can.
+//
+// The face's set-up values define three affine functions of the pixel centre P,
+// evaluated exactly: a(P) = (P.x - bx) bcny - (P.y - by) bcnx, b(P) likewise for
+// edge CA, and 1 - a - b.  The cheap test computes them in binary32 and rejects
+// when one is below -2**-40; its rounding error over the reference bbox is at
+// most e_a = 8u (|P.x - bx| |bcny| + |P.y - by| |bcnx| + the rounding of P), u =
+// 2**-24 (and e_b likewise, e_t = e_a + e_b + 4u (1 + |a| + |b|)).  So every
+// survivor lies in the triangle a >= -e_a', b >= -e_b', 1 - a - b >= -e_t' (e' =
+// e + 2**-40), whose vertices are solved here in float64 (the products of two
+// binary32 values are exact; the rest carries a relative error near 2**-50, and
+// the determinant is used only when it keeps 20 of its bits).  The pixels whose
+// centres fall inside that triangle's bbox, widened by 2**-30 of the magnitudes
+// involved, are kept.  This is the set the cheap test could accept, not the
+// triangle's hull: a face whose rounded coefficients describe a slightly
+// different triangle keeps the pixels of that one.
+__device__ __forceinline__ bool tight_bbox(const Cam& cam, FaceSetup& s) {
+  const double u = 5.9604644775390625e-8, tiny = 9.094947017729282e-13;  // 2**-24, 2**-40
+  const double bias_x = cam.bias[0], bias_y = cam.bias[1];
+  const double px0 = s.botx + bias_x, px1 = s.topx + bias_x, py0 = s.boty + bias_y, py1 = s.topy + bias_y;
+  const double bx = s.bx, by = s.by, cx = s.cx, cy = s.cy;
+  const double bcnx = s.bcnx, bcny = s.bcny, canx = s.canx, cany = s.cany;
+  // a NaN would drop out of the min / max below: keep the reference bbox
+  if (!(fabs(bx) + fabs(by) + fabs(cx) + fabs(cy) + fabs(bcnx) + fabs(bcny) + fabs(canx) + fabs(cany) < 1e300))
+    return true;
+  // |P - vertex| over the bbox, plus the rounding of P itself
+  const double rx = fmax(fabs(px0), fabs(px1)) * u, ry = fmax(fabs(py0), fabs(py1)) * u;
+  const double dbx = fmax(fabs(px0 - bx), fabs(px1 - bx)) + rx + 1.0, dby = fmax(fabs(py0 - by), fabs(py1 - by)) + ry + 1.0;
+  const double dcx = fmax(fabs(px0 - cx), fabs(px1 - cx)) + rx + 1.0, dcy = fmax(fabs(py0 - cy), fabs(py1 - cy)) + ry + 1.0;
+  const double m_a = dbx * fabs(bcny) + dby * fabs(bcnx), m_b = dcx * fabs(cany) + dcy * fabs(canx);
+  const double e_a = tiny + 8.0 * u * m_a, e_b = tiny + 8.0 * u * m_b;
+  const double e_t = tiny + e_a + e_b + 4.0 * u * (1.0 + m_a + m_b);
+  const double det = bcnx * cany - bcny * canx;
+  const double dterm = fabs(bcnx * cany) + fabs(bcny * canx);
+  if (!(fabs(det) > dterm * 9.5367431640625e-7)) return true;  // 2**-20: keep the reference bbox
+  const double inv = 1.0 / det;
+  const double A0 = bx * bcny - by * bcnx, B0 = cx * cany - cy * canx;
+  // P(a, b) solves (P.x - bx) bcny - (P.y - by) bcnx = a, (P.x - cx) cany - (P.y - cy) canx = b
+  const double av[3] = {-e_a, 1.0 + e_t + e_b, -e_a};
+  const double bv[3] = {-e_b, -e_b, 1.0 + e_t + e_a};
+  double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300, mag = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double ta = av[k] + A0, tb = bv[k] + B0;
+    const double X = (bcnx * tb - canx * ta) * inv, Y = (bcny * tb - cany * ta) * inv;
+    xmin = fmin(xmin, X); xmax = fmax(xmax, X);
+    ymin = fmin(ymin, Y); ymax = fmax(ymax, Y);
+    mag = fmax(mag, (fabs(bcnx * tb) + fabs(canx * ta) + fabs(bcny * tb) + fabs(cany * ta)) * fabs(inv) +
+                        fabs(X) + fabs(Y));
+  }
+  const double d = 9.313225746154785e-10 * (mag + 1.0);  // 2**-30
+  // keep pixel x when its centre x + bias lies in [xmin - d, xmax + d]
+  const double lx = fmax(ceil(xmin - d - bias_x), (double)s.botx), hx = fmin(floor(xmax + d - bias_x), (double)s.topx);
+  const double ly = fmax(ceil(ymin - d - bias_y), (double)s.boty), hy = fmin(floor(ymax + d - bias_y), (double)s.topy);
+  if (lx > hx || ly > hy) return false;
+  s.botx = (int32_t)lx; s.topx = (int32_t)hx;
+  s.boty = (int32_t)ly; s.topy = (int32_t)hy;
+  return true;
+}
+
 // Binning workgroup g of job j (the body of k_bin).  Needs (ntiles + bin_faces)
 // words of dynamic LDS.
 __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
@@ -160,7 +221,8 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
     uint32_t sp = 0xFFFFFFFFu;
     bool giant = false;
     FaceSetup s;
-    if (f < j.T && setup_face(cam, v, j.flags, s) && s.topx >= s.botx && s.topy >= s.boty) {
+    if (f < j.T && setup_face(cam, v, j.flags, s) && s.topx >= s.botx && s.topy >= s.boty &&
+        (!face_well_conditioned(s) || tight_bbox(cam, s))) {
       const int tx0 = s.botx >> kTileLog, tx1 = s.topx >> kTileLog;
       const int ty0 = s.boty >> kTileLog, ty1 = s.topy >> kTileLog;
       const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1;

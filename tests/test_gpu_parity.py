@@ -148,6 +148,50 @@ def test_raster_thin_and_screen_filling_faces(gpu):
             assert np.array_equal(depth.cpu().numpy(), od)
 
 
+def test_raster_slivers_and_subpixel_faces(gpu):
+    """Faces whose pixel set is decided by rounding: near-collinear slivers (the
+    third vertex 1e-6 to 1e-2 px off the opposite edge), sub-pixel triangles,
+    and triangles with vertices on pixel centres (edges through centres) -- the
+    cases where the binned path's shrunken per-face bboxes must still hold every
+    pixel the reference's coverage test accepts.  Bit-exact ids and depths."""
+    import torch
+    import taichi_three_amd as tina
+    W, H = 640, 480
+    rng = np.random.default_rng(77)
+    n = 30000
+    c = rng.uniform([-0.95, -0.95], [0.95, 0.95], (n, 2))
+    d = rng.normal(0, 0.03, (n, 2))
+    t = rng.uniform(0.1, 0.9, (n, 1))
+    perp = np.stack([-d[:, 1], d[:, 0]], 1) / np.linalg.norm(d, axis=1, keepdims=True)
+    off = (10.0 ** rng.uniform(-6, -2, (n, 1))) * (2.0 / W) * rng.choice([-1, 1], (n, 1))
+    sliver = np.stack([c, c + d, c + t * d + off * perp], 1)
+    tiny = c[:, None, :] + rng.normal(0, 0.6 / W, (n, 3, 2))
+    grid = (np.floor((c[:, None, :] * 0.5 + 0.5) * [W, H]) + rng.integers(-4, 5, (n, 3, 2)) + 0.5) / [W, H] * 2 - 1
+    xy = np.concatenate([sliver, tiny, grid])
+    z = rng.uniform(-0.9, 0.9, xy.shape[:2] + (1,))
+    verts = np.concatenate([xy, z], -1).astype(np.float32)
+    M = np.diag([1.0, 1.0, -1.0, 1.0]).astype(np.float32)
+    for culling in (True, False):
+        print(f'case: slivers and sub-pixel faces culling={culling}', flush=True)
+        eng = tina.Engine(W, H, maxfaces=len(verts))
+        eng.W2V = M
+        eng.V2W = np.linalg.inv(M).astype(np.float32)
+        eng._push_camera()
+        r = tina.TriangleRaster(eng, maxfaces=len(verts), culling=culling)
+        m = tina.SimpleMesh(maxfaces=len(verts))
+        m.set_face_verts(verts)
+        eng.clear_depth()
+        r.set_object(m)
+        r.render_occup()
+        ids, depth = eng.resolve()
+        torch.cuda.synchronize()
+        cam = orc.camera(eng.W2V, eng.V2W)
+        od, oi = orc.raster(W, H, cam, verts, flags=(1 if culling else 0) | 2)
+        assert (oi >= 0).sum() > 10000
+        assert np.array_equal(ids.cpu().numpy(), oi), f'{(ids.cpu().numpy() != oi).sum()} id mismatches'
+        assert np.array_equal(depth.cpu().numpy(), od)
+
+
 @pytest.mark.parametrize('seed', [31, 32])
 def test_raster_camera_inside_the_cloud(gpu, seed):
     """Camera in the middle of the soup: faces next to the eye (1/w far from 1,
