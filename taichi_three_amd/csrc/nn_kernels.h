@@ -36,6 +36,11 @@ constexpr float kErrWeight = 1.0f;
 constexpr int kNnFrags = 22;
 constexpr int kNnPackBias = kNnFrags * 64 * 8;  // offset of the biases, in bf16 units
 constexpr int kNnPackBytes = kNnFrags * 64 * 16 + (3 * 64 + 4) * 4;
+// Fallback queue of k_shade_nn: pixel chunk c appends to segment c % kFbSeg,
+// whose counter sits alone on a 128-byte line (one global atomic per chunk).
+// Per buffer: kFbSeg counters kFbLine words apart, then the last pass's total.
+constexpr int kFbSeg = 32, kFbLine = 32;
+constexpr int kFbWords = (kFbSeg + 1) * kFbLine;
 
 // ---- random inputs -----------------------------------------------------------
 
@@ -388,27 +393,40 @@ __device__ __forceinline__ bf16x8 nn_next_b(const f32x4& lo, const f32x4& hi) {
   return __builtin_bit_cast(bf16x8, r);
 }
 
-// This lane's A-operand fragments of the four layers (the biases live in LDS).
-struct NnWeights {
-  bf16x8 w1[4], w2[8], w3[8], w4[2];
-};
-
-// The analytic shader over the queued pixels fbq[0 .. *fbn); also clears
-// *fbn_next, the counter the next k_shade_nn launch appends to.
+// The analytic shader over the pixels k_shade_nn queued: segment k holds
+// fbn[k * kFbLine] pixels at fbq + k * segcap.  Also clears fbn_next, the
+// counters the next k_shade_nn launch appends to, and stores the total after
+// the counters (fbn[kFbSeg * kFbLine]).
 template <int PROG, bool IBL>
 __global__ void __launch_bounds__(kBlock)
 k_shade_list(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ fbq,
-             const uint32_t* __restrict__ fbn, uint32_t* __restrict__ fbn_next, float* __restrict__ img) {
+             long segcap, uint32_t* __restrict__ fbn, uint32_t* __restrict__ fbn_next, float* __restrict__ img) {
   (void)a_by_value;
 #if defined(__HIP_DEVICE_COMPILE__)
   const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
 #else
   const ShadeArgs& a = a_by_value;
 #endif
-  if (blockIdx.x == 0 && threadIdx.x == 0) *fbn_next = 0u;
-  const uint32_t cnt = *fbn;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock) {
-    const long P = fbq[i];
+  __shared__ uint32_t s_pre[kFbSeg + 1];  // s_pre[k]: queued pixels in segments before k
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const uint32_t cnt = tid < kFbSeg ? fbn[tid * kFbLine] : 0u;
+    const int incl = wave_inclusive_scan((int)cnt, tid);
+    if (tid < kFbSeg) s_pre[tid + 1] = (uint32_t)incl;
+    if (tid == 0) s_pre[0] = 0;
+  }
+  __syncthreads();
+  const uint32_t total = s_pre[kFbSeg];
+  if (blockIdx.x == 0) {
+    if (tid < kFbSeg) fbn_next[tid * kFbLine] = 0u;
+    if (tid == 0) fbn[kFbSeg * kFbLine] = total;
+  }
+  for (uint32_t i = blockIdx.x * kBlock + tid; i < total; i += gridDim.x * kBlock) {
+    int k = 0;  // s_pre[k] <= i < s_pre[k + 1]
+#pragma unroll
+    for (int step = kFbSeg / 2; step > 0; step >>= 1)
+      if (s_pre[k + step] <= i) k += step;
+    const long P = fbq[k * segcap + (i - s_pre[k])];
     const uint32_t f = (uint32_t)keys[P] - a.mesh.id_base;
     const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
     store_rgb(img, P, shade_pixel<PROG, IBL>(a, f, px, py));

@@ -75,9 +75,15 @@ __device__ __forceinline__ void nn_inputs(const ShadeArgs& a, uint32_t f, int px
 
 // The network on four 16-pixel blocks at once, layer by layer, so that every
 // layer issues 16 independent MFMAs (4 row blocks x 4 pixel blocks) back to
-// back instead of one dependent chain per block.
-__device__ __forceinline__ void nn_eval64(const NnWeights& w, const float* __restrict__ sb, int g,
-                                          const bf16x8 (&x)[4], f32x4 (&o)[4]) {
+// back instead of one dependent chain per block.  The 22 weight fragments are
+// read from LDS (sw: fragment q of this lane at sw[64 q + lane]) where they are
+// used: kept in VGPRs they cost 88 registers a lane and the occupancy that
+// hides the per-pixel gathers.  `lane` is laundered by the caller, so the
+// reads stay inside its pixel loop.
+__device__ __forceinline__ void nn_eval64(const bf16x8* __restrict__ sw, uint32_t lane,
+                                          const float* __restrict__ sb, int g, const bf16x8 (&x)[4],
+                                          f32x4 (&o)[4]) {
+  auto wf = [&](int q) { return sw[q * 64 + lane]; };
   auto bias = [&](int base, int mb) { return *(const f32x4*)(sb + base + 16 * mb + 4 * g); };
   f32x4 h[4][4];
   bf16x8 k0[4], k1[4];
@@ -85,7 +91,7 @@ __device__ __forceinline__ void nn_eval64(const NnWeights& w, const float* __res
   for (int mb = 0; mb < 4; mb++) {
     const f32x4 b = bias(0, mb);
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w1[mb], x[nb], b, 0, 0, 0);
+    for (int nb = 0; nb < 4; nb++) h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf(mb), x[nb], b, 0, 0, 0);
   }
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
@@ -96,10 +102,10 @@ __device__ __forceinline__ void nn_eval64(const NnWeights& w, const float* __res
   for (int mb = 0; mb < 4; mb++) {
     const f32x4 b = bias(64, mb);
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb], k0[nb], b, 0, 0, 0);
+    for (int nb = 0; nb < 4; nb++) h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf(4 + 2 * mb), k0[nb], b, 0, 0, 0);
 #pragma unroll
     for (int nb = 0; nb < 4; nb++)
-      h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w2[2 * mb + 1], k1[nb], h[nb][mb], 0, 0, 0);
+      h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf(5 + 2 * mb), k1[nb], h[nb][mb], 0, 0, 0);
   }
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
@@ -110,10 +116,10 @@ __device__ __forceinline__ void nn_eval64(const NnWeights& w, const float* __res
   for (int mb = 0; mb < 4; mb++) {
     const f32x4 b = bias(128, mb);
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb], k0[nb], b, 0, 0, 0);
+    for (int nb = 0; nb < 4; nb++) h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf(12 + 2 * mb), k0[nb], b, 0, 0, 0);
 #pragma unroll
     for (int nb = 0; nb < 4; nb++)
-      h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w3[2 * mb + 1], k1[nb], h[nb][mb], 0, 0, 0);
+      h[nb][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf(13 + 2 * mb), k1[nb], h[nb][mb], 0, 0, 0);
   }
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
@@ -122,36 +128,25 @@ __device__ __forceinline__ void nn_eval64(const NnWeights& w, const float* __res
   }
   const f32x4 b4 = bias(192, 0);
 #pragma unroll
-  for (int nb = 0; nb < 4; nb++) o[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[0], k0[nb], b4, 0, 0, 0);
+  for (int nb = 0; nb < 4; nb++) o[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf(20), k0[nb], b4, 0, 0, 0);
 #pragma unroll
-  for (int nb = 0; nb < 4; nb++) o[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.w4[1], k1[nb], o[nb], 0, 0, 0);
+  for (int nb = 0; nb < 4; nb++) o[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf(21), k1[nb], o[nb], 0, 0, 0);
 }
 
-constexpr int kNnShadeGrid = 1024;  // persistent workgroups of k_shade_nn
-
-// Loads the packed weight fragments into this lane's registers and pins them
-// there (otherwise the compiler re-reads them inside the pixel loop), and the
-// biases into LDS (b1, b2, b3, then b4 padded to 16 rows).
-__device__ __forceinline__ void nn_resident(const __bf16* __restrict__ pk, int lane, NnWeights& w, float* s_b) {
+// Copies the packed network into LDS: the 22 weight fragments (sw) and the
+// biases b1, b2, b3, then b4 padded to 16 rows (sb).
+__device__ __forceinline__ void nn_stage(const __bf16* __restrict__ pk, bf16x8* sw, float* sb) {
   const float* bias = (const float*)(pk + kNnPackBias);
-  for (int i = threadIdx.x; i < 3 * 64 + 16; i += blockDim.x) s_b[i] = i < 3 * 64 + kNnOut ? bias[i] : 0.0f;
+  for (int i = threadIdx.x; i < 3 * 64 + 16; i += blockDim.x) sb[i] = i < 3 * 64 + kNnOut ? bias[i] : 0.0f;
   const bf16x8* fr = (const bf16x8*)pk;
-#pragma unroll
-  for (int i = 0; i < 4; i++) w.w1[i] = fr[i * 64 + lane];
-#pragma unroll
-  for (int i = 0; i < 8; i++) w.w2[i] = fr[(4 + i) * 64 + lane];
-#pragma unroll
-  for (int i = 0; i < 8; i++) w.w3[i] = fr[(12 + i) * 64 + lane];
-#pragma unroll
-  for (int i = 0; i < 2; i++) w.w4[i] = fr[(20 + i) * 64 + lane];
-#pragma unroll
-  for (int i = 0; i < 4; i++) asm volatile("" : "+v"(w.w1[i]));
-#pragma unroll
-  for (int i = 0; i < 8; i++) asm volatile("" : "+v"(w.w2[i]));
-#pragma unroll
-  for (int i = 0; i < 8; i++) asm volatile("" : "+v"(w.w3[i]));
-#pragma unroll
-  for (int i = 0; i < 2; i++) asm volatile("" : "+v"(w.w4[i]));
+  for (int i = threadIdx.x; i < kNnFrags * 64; i += blockDim.x) sw[i] = fr[i];
+}
+
+// This lane's index, opaque to the compiler: keeps LDS weight reads inside a loop.
+__device__ __forceinline__ uint32_t laundered_lane(int lane) {
+  uint32_t l = (uint32_t)lane;
+  asm volatile("" : "+v"(l));
+  return l;
 }
 
 // The network on n feature rows X[n][12] -> O[n][4] (max(0, colour), error
@@ -164,9 +159,9 @@ __device__ __forceinline__ void nn_resident(const __bf16* __restrict__ pk, int l
 __global__ void __launch_bounds__(kBlock)
 k_mlp_rows(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, float* __restrict__ O) {
   __shared__ __attribute__((aligned(16))) float s_b[3 * 64 + 16];
+  __shared__ __attribute__((aligned(16))) bf16x8 s_w[kNnFrags * 64];
   const int lane = __lane_id(), wv = threadIdx.x / 64, g = lane / 16, j = lane % 16;
-  NnWeights w;
-  nn_resident(pk, lane, w, s_b);
+  nn_stage(pk, s_w, s_b);
   __syncthreads();
   const long nchunks = (n + 63) / 64;
   for (long c = (long)blockIdx.x * (kBlock / 64) + wv; c < nchunks; c += (long)gridDim.x * (kBlock / 64)) {
@@ -184,7 +179,7 @@ k_mlp_rows(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, f
       const u32x4 v = {bf16_pair(p.x, p.y), bf16_pair(p.z, p.w), bf16_pair(q.x, q.y), bf16_pair(q.z, q.w)};
       xb[nb] = __builtin_bit_cast(bf16x8, v);
     }
-    nn_eval64(w, s_b, g, xb, ob);
+    nn_eval64(s_w, laundered_lane(lane), s_b, g, xb, ob);
     if (g == 0) {
 #pragma unroll
       for (int nb = 0; nb < 4; nb++) {
@@ -199,7 +194,8 @@ k_mlp_rows(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, f
 
 __global__ void __launch_bounds__(kBlock)
 k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const __bf16* __restrict__ pk,
-           float threshold, uint32_t* __restrict__ fbq, uint32_t* __restrict__ fbn, float* __restrict__ img) {
+           float threshold, uint32_t* __restrict__ fbq, long segcap, uint32_t* __restrict__ fbn,
+           float* __restrict__ img) {
   (void)a_by_value;
 #if defined(__HIP_DEVICE_COMPILE__)
   const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -208,25 +204,28 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
 #endif
   __shared__ uint32_t s_pix[kShadePix];
   __shared__ uint32_t s_face[kShadePix];
-  __shared__ uint32_t s_n;
+  __shared__ uint32_t s_fb[kShadePix];  // the chunk's pixels for the analytic kernel
+  __shared__ uint32_t s_n, s_fbn, s_fbbase;
   __shared__ __attribute__((aligned(16))) __bf16 s_x[kBlock / 64][64][16];
   __shared__ __attribute__((aligned(16))) float s_o[kBlock / 64][64][4];
-  // biases b1, b2, b3 (64 each), then b4 padded to 16 rows
   __shared__ __attribute__((aligned(16))) float s_b[3 * 64 + 16];
+  __shared__ __attribute__((aligned(16))) bf16x8 s_w[kNnFrags * 64];
   const int tid = threadIdx.x;
   const int lane = __lane_id(), wv = tid / 64, g = lane / 16, j = lane % 16;
   const long npix = (long)a.cam.W * a.cam.H;
   const long nchunks = (npix + kShadePix - 1) / kShadePix;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  NnWeights w;
-  nn_resident(pk, lane, w, s_b);
+  nn_stage(pk, s_w, s_b);
   const tina_material& m = a.mat;
   const int fres = m.ops[2].b;  // FresnelFactor slot: metallic = its first operand
   const V3 L = light_at(a.lights, 0).xyz;
   for (long chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const long base = chunk * kShadePix;
-    __syncthreads();  // the previous chunk's list is consumed
-    if (tid == 0) s_n = 0;
+    __syncthreads();  // the previous chunk's lists are consumed
+    if (tid == 0) {
+      s_n = 0;
+      s_fbn = 0;
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kShadePix / kBlock; r++) {
@@ -299,7 +298,7 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
           for (int e = 0; e < 8; e++) xb[nb][e] = (__bf16)0.0f;
         }
       }
-      nn_eval64(w, s_b, g, xb, ob);
+      nn_eval64(s_w, laundered_lane(lane), s_b, g, xb, ob);
       if (g == 0) {
 #pragma unroll
         for (int nb = 0; nb < 4; nb++) *(f32x4*)s_o[wv][16 * nb + j] = ob[nb];
@@ -310,13 +309,22 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
         const bool fall = !(o[3] <= threshold);
         const unsigned long long fm = __ballot(fall);
         uint32_t at = 0;
-        if (lane == 0 && fm) at = atomicAdd(fbn, (uint32_t)__popcll(fm));
+        if (lane == 0 && fm) at = atomicAdd(&s_fbn, (uint32_t)__popcll(fm));
         at = __shfl(at, 0);
-        if (fall) fbq[at + (uint32_t)__popcll(fm & lt_mask)] = (uint32_t)P;
+        if (fall) s_fb[at + (uint32_t)__popcll(fm & lt_mask)] = (uint32_t)P;
         else store_rgb(img, P, v3(fmaxf(o[0], 0.0f), fmaxf(o[1], 0.0f), fmaxf(o[2], 0.0f)));
       }
       wave_sync_lds();
     }
+    // hand the chunk's fallback pixels to its queue segment with one global atomic
+    __syncthreads();
+    const uint32_t nfb = s_fbn;
+    if (nfb == 0) continue;
+    const long seg = chunk % kFbSeg;
+    if (tid == 0) s_fbbase = atomicAdd(fbn + seg * kFbLine, nfb);
+    __syncthreads();
+    uint32_t* __restrict__ q = fbq + seg * segcap + s_fbbase;
+    for (uint32_t i = tid; i < nfb; i += kBlock) q[i] = s_fb[i];
   }
 }
 

@@ -36,10 +36,13 @@ struct tina_ctx {
   uint32_t* entries = nullptr;
   FaceRec* recs = nullptr;
   BinCounters* binctr = nullptr;
-  // learned-shading fallback queue ([H][W] pixel indices) and its two counters
+  // learned-shading fallback queue (kFbSeg segments of fb_segcap pixel indices),
+  // two buffers of segment counters, and k_shade_nn's resident grid
   uint32_t* fbq = nullptr;
   uint32_t* fbn = nullptr;
+  long fb_segcap = 0;
   int fb_which = 0;
+  int nn_grid = 0;
   tina_camera cam{};
   std::string err;
 };
@@ -57,6 +60,21 @@ int fail(tina_ctx* c, int code, const std::string& msg) {
 int hip_check(tina_ctx* c, hipError_t e, const char* what) {
   if (e == hipSuccess) return TINA_OK;
   return fail(c, TINA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Workgroups of `kernel` (block threads, no dynamic LDS) that the current device
+// holds at once: the grid of a persistent kernel, so that no workgroup waits for
+// another to finish.  `fallback` if the runtime cannot say.
+int resident_grid(const void* kernel, int block, int fallback) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || ncu <= 0 ||
+      per_cu <= 0) {
+    (void)hipGetLastError();
+    return fallback;
+  }
+  return ncu * per_cu;
 }
 
 #define TINA_HIP(ctx, call)                                   \
@@ -225,16 +243,20 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
       if (p) (void)hipFree(p);
     delete c;
   };
+  // segment k takes the fallback pixels of chunks k, k + kFbSeg, ...
+  const long nchunks = ((long)width * height + kShadePix - 1) / kShadePix;
+  c->fb_segcap = (nchunks + kFbSeg - 1) / kFbSeg * kShadePix;
   if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
       hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, sizeof(BinCounters)) != hipSuccess ||
-      hipMalloc(&c->fbq, sizeof(uint32_t) * (size_t)width * height) != hipSuccess ||
-      hipMalloc(&c->fbn, 2 * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&c->fbq, sizeof(uint32_t) * (size_t)c->fb_segcap * kFbSeg) != hipSuccess ||
+      hipMalloc(&c->fbn, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess) {
     (void)hipGetLastError();
     release();
     return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the depth|id buffer failed");
   }
   if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess ||
-      hipMemset(c->binctr, 0, sizeof(BinCounters)) != hipSuccess || hipMemset(c->fbn, 0, 2 * sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(c->binctr, 0, sizeof(BinCounters)) != hipSuccess ||
+      hipMemset(c->fbn, 0, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     release();
     return fail(nullptr, TINA_ERR_HIP, "counter initialisation failed");
@@ -245,6 +267,7 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
     release();
     return rc;
   }
+  c->nn_grid = resident_grid((const void*)k_shade_nn, kBlock, 768);
   *out = c;
   return TINA_OK;
 }
@@ -483,7 +506,8 @@ int tina_mlp_inference(const void* d_packed, const float* d_x, int64_t n, float*
   if (!d_x || !d_out) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: buffer is NULL");
   if (((uintptr_t)d_x | (uintptr_t)d_out) & 15)
     return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_inference: rows must be 16-byte aligned");
-  hipLaunchKernelGGL(k_mlp_rows, dim3(grid_for(n, kBlock, kNnShadeGrid)), dim3(kBlock), 0, (hipStream_t)stream,
+  static const int rows_grid = resident_grid((const void*)k_mlp_rows, kBlock, 768);
+  hipLaunchKernelGGL(k_mlp_rows, dim3(grid_for(n, kBlock, (unsigned)rows_grid)), dim3(kBlock), 0, (hipStream_t)stream,
                      (const __bf16*)d_packed, d_x, (long)n, d_out);
   return hip_check(nullptr, hipGetLastError(), "k_mlp_rows");
 }
@@ -513,19 +537,21 @@ int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, 
   const int which = c->fb_which;
   c->fb_which ^= 1;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kShadePix, kNnShadeGrid)), dim3(kBlock), 0, s, a, c->keys,
-                     (const __bf16*)d_packed, threshold, c->fbq, c->fbn + which, d_img);
+  uint32_t* fbn = c->fbn + which * kFbWords;
+  uint32_t* fbn_next = c->fbn + (which ^ 1) * kFbWords;
+  hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kShadePix, (unsigned)c->nn_grid)), dim3(kBlock), 0, s, a,
+                     c->keys, (const __bf16*)d_packed, threshold, c->fbq, c->fb_segcap, fbn, d_img);
   TINA_LAUNCH(c, "k_shade_nn");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade_list<PROG_PBR, true>), dim3(grid_for(npix, kBlock, 512)), dim3(kBlock), 0,
-                     s, a, c->keys, c->fbq, c->fbn + which, c->fbn + (which ^ 1), d_img);
+                     s, a, c->keys, c->fbq, c->fb_segcap, fbn, fbn_next, d_img);
   TINA_LAUNCH(c, "k_shade_list");
   return TINA_OK;
 }
 
 int tina_nn_fallback_count(tina_ctx* c, uint32_t* d_count, void* stream) {
   if (!c || !d_count) return fail(c, TINA_ERR_INVALID, "ctx or out is NULL");
-  TINA_HIP(c, hipMemcpyAsync(d_count, c->fbn + (c->fb_which ^ 1), sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                             (hipStream_t)stream));
+  TINA_HIP(c, hipMemcpyAsync(d_count, c->fbn + (c->fb_which ^ 1) * kFbWords + kFbSeg * kFbLine, sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return TINA_OK;
 }
 
