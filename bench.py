@@ -111,17 +111,18 @@ def eval_only(ns, n, reps):
     import torch
     X, _ = ns.samples(n, seed=7)
 
-    def timed(fn):
+    def timed(fn, calls=10):  # median over reps of the mean of `calls` back-to-back calls
         fn(X)
         torch.cuda.synchronize()
         ts = []
         for _ in range(reps):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-            fn(X)
+            for _ in range(calls):
+                fn(X)
             e1.record()
             torch.cuda.synchronize()
-            ts.append(e0.elapsed_time(e1))
+            ts.append(e0.elapsed_time(e1) / calls)
         return float(np.median(ts))
 
     t_nn, t_an = timed(ns.infer), timed(ns.analytic)
@@ -220,7 +221,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-neural', action='store_true')
     ap.add_argument('--no-batch', action='store_true')
-    ap.add_argument('--train-steps', type=int, default=20000)
+    ap.add_argument("--train-steps", type=int, default=60000)
     ap.add_argument('--nfaces', type=int, default=2_000_000)
     ap.add_argument('--copy-mb', type=int, default=0,
                     help='first run one device copy of this many MiB (a known-size kernel that '

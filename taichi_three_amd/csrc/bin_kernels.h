@@ -81,6 +81,7 @@ struct BinJob {
   uint32_t id_base, flags;
   int tiles_x, ntiles, ngroups;
   int bin_faces, max_tiles;
+  int clear_keys;  // k_bin first resets the W x H key buffer (a clear deferred to this call)
 };
 
 // The tiles a face's bbox touches: x0 (12 bits), y0 (12 bits), x span - 1 (4 bits),
@@ -479,6 +480,13 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
 
 __global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
+  if (j.clear_keys) {  // nothing in this kernel reads the keys; k_giant and k_tile run after it
+    const long n = (long)j.cam.W * j.cam.H, stride = (long)gridDim.x * kBinBlock;
+    const uint64_t e = empty_key();
+    ulonglong2* k2 = reinterpret_cast<ulonglong2*>(j.keys);
+    for (long i = (long)blockIdx.x * kBinBlock + threadIdx.x; i < (n >> 1); i += stride) k2[i] = make_ulonglong2(e, e);
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) j.keys[n - 1] = e;
+  }
   bin_group(j, blockIdx.x);
 }
 

@@ -43,6 +43,10 @@ struct tina_ctx {
   long fb_segcap = 0;
   int fb_which = 0;
   int nn_grid = 0;
+  // tina_clear only records the clear; the next call that touches the keys applies it
+  // (the binned raster inside k_bin, anything else with k_clear first)
+  bool clear_pending = false;
+  hipStream_t clear_stream = nullptr;
   tina_camera cam{};
   std::string err;
 };
@@ -106,6 +110,16 @@ unsigned grid_for(long n, long per_block, unsigned cap) {
   if (g > (long)cap) g = cap;
   return (unsigned)g;
 }
+
+// Applies a clear recorded by tina_clear, on `stream`.
+int flush_clear(tina_ctx* c, hipStream_t stream) {
+  if (!c->clear_pending) return TINA_OK;
+  c->clear_pending = false;
+  const long n = (long)c->W * c->H;
+  hipLaunchKernelGGL(k_clear, dim3(grid_for(n / 2 + 1, kBlock, 4096)), dim3(kBlock), 0, stream, c->keys, n);
+  return hip_check(c, hipGetLastError(), "k_clear");
+}
+
 
 void free_scratch(tina_ctx* c) {
   for (void* p : {(void*)c->bigq, (void*)c->giantq, (void*)c->chunk_off, (void*)c->chunk_cnt, (void*)c->entries, (void*)c->recs})
@@ -297,16 +311,16 @@ int tina_set_camera(tina_ctx* c, const tina_camera* cam) {
 
 int tina_keys(tina_ctx* c, uint64_t** d_keys) {
   if (!c || !d_keys) return fail(c, TINA_ERR_INVALID, "ctx or out is NULL");
+  int rc = flush_clear(c, c->clear_stream);
+  if (rc) return rc;
   *d_keys = c->keys;
   return TINA_OK;
 }
 
 int tina_clear(tina_ctx* c, void* stream) {
   if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
-  const long n = (long)c->W * c->H;
-  hipLaunchKernelGGL(k_clear, dim3(grid_for(n / 2 + 1, kBlock, 4096)), dim3(kBlock), 0, (hipStream_t)stream,
-                     c->keys, n);
-  TINA_LAUNCH(c, "k_clear");
+  c->clear_pending = true;
+  c->clear_stream = (hipStream_t)stream;
   return TINA_OK;
 }
 
@@ -341,15 +355,20 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.bin_faces = bin_faces_for(j.ntiles);
     j.ngroups = (nfaces + j.bin_faces - 1) / j.bin_faces;
     j.max_tiles = kBinMaxTiles;
+    j.clear_keys = c->clear_pending ? 1 : 0;
+    c->clear_pending = false;
     hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + j.bin_faces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
-    hipLaunchKernelGGL(k_giant, dim3(kQueueGrid), dim3(kBlock), 0, s, j);
+    // usually no face is giant: a quarter of the queue grid dispatches in a quarter of the time
+    hipLaunchKernelGGL(k_giant, dim3(kQueueGrid / 4), dim3(kBlock), 0, s, j);
     TINA_LAUNCH(c, "k_giant");
     hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), (size_t)(j.ngroups + 1) * 4, s, j);
     TINA_LAUNCH(c, "k_tile");
     return TINA_OK;
   }
   // resolutions or face counts beyond the binned path's LDS tables: global atomics
+  int rc = flush_clear(c, s);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_raster_small, dim3(grid_for(nfaces, kBlock, 0x7fffffff)), dim3(kBlock), 0, s, d_verts,
                      nfaces, id_base, flags, cam, c->keys, c->bigq, c->giantq, c->ctr);
   TINA_LAUNCH(c, "k_raster_small");
@@ -373,6 +392,7 @@ int tina_shade(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, con
   ShadeArgs a;
   rc = make_shade_args(c, mesh, mat, lights, ibl, bg, flags, a);
   if (rc) return rc;
+  if ((rc = flush_clear(c, (hipStream_t)stream))) return rc;
   const long npix = (long)c->W * c->H;
   const bool ibl_mode = (flags & TINA_IBL) != 0;
   dispatch_shade(classify_program(a.mat, ibl_mode), ibl_mode, grid_for(npix, kShadePix, 0x7fffffff),
@@ -533,6 +553,7 @@ int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, 
   ShadeArgs a;
   rc = make_shade_args(c, mesh, mat, lights, ibl, bg, flags, a);
   if (rc) return rc;
+  if ((rc = flush_clear(c, (hipStream_t)stream))) return rc;
   const long npix = (long)c->W * c->H;
   const int which = c->fb_which;
   c->fb_which ^= 1;
@@ -593,6 +614,7 @@ int tina_dshade_forward(tina_ctx* c, const tina_dmesh* m, const tina_lights* lig
   DiffArgs A;
   int rc = make_diff_args(c, m, lights, A);
   if (rc) return rc;
+  if ((rc = flush_clear(c, (hipStream_t)stream))) return rc;
   const long npix = (long)c->W * c->H;
   hipLaunchKernelGGL(k_dshade_fwd, dim3(grid_for(npix, kBlock, 0x7fffffff)), dim3(kBlock), 0, (hipStream_t)stream,
                      A, c->keys, d_img);
@@ -608,6 +630,7 @@ int tina_dshade_backward(tina_ctx* c, const tina_dmesh* m, const tina_lights* li
   DiffArgs A;
   int rc = make_diff_args(c, m, lights, A);
   if (rc) return rc;
+  if ((rc = flush_clear(c, (hipStream_t)stream))) return rc;
   const long npix = (long)c->W * c->H;
   hipLaunchKernelGGL(k_dshade_bwd, dim3(grid_for(npix, kBlock, 0x7fffffff)), dim3(kBlock), 0, (hipStream_t)stream,
                      A, c->keys, d_gimg, (float*)g->colors, (float*)g->normals, (float*)g->rough, (float*)g->metal,
@@ -733,6 +756,7 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     j.ngroups = ngroups;
     j.bin_faces = bf;
     j.max_tiles = tune->max_tiles;
+    j.clear_keys = 0;
     ShadeArgs& a = sh[i].a;
     memset(&a, 0, sizeof(a));
     a.mesh = d.mesh;
@@ -862,6 +886,8 @@ int tina_resolve_keys(const uint64_t* d_keys, int64_t n, int32_t* d_id, int32_t*
 int tina_resolve(tina_ctx* c, int32_t* d_id, int32_t* d_depth, void* stream) {
   if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
   if (!d_id && !d_depth) return TINA_OK;
+  int rc = flush_clear(c, (hipStream_t)stream);
+  if (rc) return rc;
   const long n = (long)c->W * c->H;
   hipLaunchKernelGGL(k_resolve, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, c->keys,
                      d_id, d_depth, n);

@@ -135,7 +135,9 @@ class Engine:
         return Field(self.resolve(want_depth=False)[0])
 
     def keys(self):
-        """The packed u64 key buffer as an int64 tensor view [H, W] (no copy)."""
+        """The packed u64 key buffer as an int64 tensor view [H, W] (no copy).
+        A clear is applied by the next library call that touches the buffer
+        (this one included), so take the view after the clear you want to see."""
         p = ctypes.c_void_p()
         _abi.check(self._lib.tina_keys(self._ctx, ctypes.byref(p)), self._ctx)
         H, W = int(self.res[1]), int(self.res[0])
