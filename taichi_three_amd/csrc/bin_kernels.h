@@ -37,8 +37,10 @@ constexpr int kBinMaxTiles = 64;      // bbox tiles above which a face is raster
 // binning workgroups a k_tile workgroup can index: its g + 1 words of dynamic
 // LDS plus ~21 KB of static LDS stay within a 64 KB workgroup allocation
 constexpr int kMaxBinGroups = 8192;
-// bytes of dynamic LDS a binning workgroup may take (its static LDS is 72 bytes)
-constexpr int kLdsBudget = 65536 - 128;
+// bytes of dynamic LDS a binning workgroup may take (its static LDS is 72 bytes):
+// a gfx950 CU has 160 KB of LDS and one workgroup may use all of it, so screens
+// up to ~40k tiles (3840x2160 is 32,400) stay on the binned path
+constexpr int kLdsBudget = 160 * 1024 - 1024;
 
 // Faces per binning workgroup for a screen of ntiles tiles: the largest of 8192,
 // 4096, 2048 and 1024 whose LDS tables (ntiles + faces words) fit the budget,

@@ -111,6 +111,17 @@ unsigned grid_for(long n, long per_block, unsigned cap) {
   return (unsigned)g;
 }
 
+// Lets the binning kernels take more than 64 KB of dynamic LDS (once per process).
+void allow_big_lds() {
+  static const bool done = [] {
+    (void)hipFuncSetAttribute((const void*)k_bin, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBudget);
+    (void)hipFuncSetAttribute((const void*)k_bbin, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBudget);
+    (void)hipGetLastError();
+    return true;
+  }();
+  (void)done;
+}
+
 // Applies a clear recorded by tina_clear, on `stream`.
 int flush_clear(tina_ctx* c, hipStream_t stream) {
   if (!c->clear_pending) return TINA_OK;
@@ -357,6 +368,7 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.max_tiles = kBinMaxTiles;
     j.clear_keys = c->clear_pending ? 1 : 0;
     c->clear_pending = false;
+    allow_big_lds();
     hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + j.bin_faces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
     // usually no face is giant: a quarter of the queue grid dispatches in a quarter of the time
@@ -816,6 +828,7 @@ int tina_batch_render(tina_batch* b, void* stream) {
                      b->total_pix);
   int rc = hip_check(nullptr, hipGetLastError(), "k_clear");
   if (rc) return rc;
+  allow_big_lds();
   hipLaunchKernelGGL(k_bbin, dim3(batch_grid(b->tune.grid_bin, b->totals[0])), dim3(kBinBlock),
                      (size_t)(b->max_ntiles + b->tune.bin_faces) * 4, s, b->d_jobs, b->d_prefix, n);
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_bbin"))) return rc;

@@ -30,10 +30,11 @@ def _raster_vs_oracle(W, H, verts, culling=True):
     return ids.cpu().numpy(), depth.cpu().numpy(), oi, od
 
 
-@pytest.mark.parametrize('W,H', [(1, 1), (1, 700), (700, 1), (17, 3), (2200, 2000)])
+@pytest.mark.parametrize('W,H', [(1, 1), (1, 700), (700, 1), (17, 3), (3840, 2160), (4000, 2600)])
 def test_degenerate_and_oversized_screens(gpu, W, H):
-    """(2200, 2000) has 17,250 tiles: more than the binned path's LDS tables hold,
-    so it runs the global-atomic path."""
+    """(3840, 2160) has 32,400 tiles, which the binned path holds only with the
+    160 KB LDS of a gfx950 workgroup; (4000, 2600) has 40,750, more than its LDS
+    tables hold, so it runs the global-atomic path."""
     from taichi_three_amd import workloads as wl
     rng = np.random.default_rng(W * 7 + H)
     verts = wl.triangle_soup(rng, 20000, np.log(0.05), 0.8)
