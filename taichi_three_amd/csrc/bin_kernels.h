@@ -280,8 +280,7 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
 // Faces too large to bin: every lane of the grid takes a slice of each one and
 // reduces into the key buffer with global atomics.  Runs between k_bin and
 // k_tile (k_tile loads the keys after it and resets the queue).
-__device__ __forceinline__ void giant_body(const BinJob& j) {
-  const uint32_t ng = j.ctr->giant;
+__device__ __forceinline__ void giant_body(const BinJob& j, uint32_t ng) {
   const long stride = (long)gridDim.x * kBlock;
   for (uint32_t r = 0; r < ng; r++) {
     const BigRec rec = j.giantq[r];
@@ -494,7 +493,7 @@ __global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
 
 __global__ void __launch_bounds__(kBlock) k_giant(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
-  giant_body(j);
+  giant_body(j, j.ctr->giant);
 }
 
 // Workgroup b of n -> tile.  The dispatcher deals workgroups round-robin over
@@ -545,8 +544,14 @@ k_bbin(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njob
   }
 }
 
+// The giant counts of all scenes are read at once (a batch holds at most kBlock
+// scenes) rather than one dependent load per scene.
 __global__ void __launch_bounds__(kBlock) k_bgiant(const BinJob* __restrict__ jobs, int njobs) {
-  for (int s = 0; s < njobs; s++) giant_body(jobs[s]);
+  __shared__ uint32_t s_ng[kBlock];
+  if (threadIdx.x < njobs) s_ng[threadIdx.x] = jobs[threadIdx.x].ctr->giant;
+  __syncthreads();
+  for (int s = 0; s < njobs; s++)
+    if (s_ng[s]) giant_body(jobs[s], s_ng[s]);
 }
 
 __global__ void __launch_bounds__(kBlock)

@@ -1,12 +1,13 @@
 // nn_shade.h -- deferred shading through the learned surrogate (k_shade_nn).
 //
-// Persistent workgroups: each loads the packed network once (A-operand
-// fragments in VGPRs, biases in LDS) and then walks kShadePix-pixel chunks.
+// Persistent workgroups: each stages the packed network in LDS once (A-operand
+// fragments and biases) and then walks kNnShadePix-pixel chunks.
 // Per chunk it paints the empty pixels with the background, compacts the
 // object's covered pixels into an LDS list with a wave ballot, and each wave
 // takes the list 64 entries at a time: network inputs per lane, the network on
 // four 16-pixel MFMA blocks, the ACES colour stored where the error estimate is
-// at most `threshold` and the pixel queued for the analytic kernel otherwise.
+// at most `threshold` and the pixel queued for the analytic kernel otherwise
+// (per chunk in LDS, then one global atomic reserves its queue segment).
 //
 // The inputs are computed with v_rcp / v_rsq in place of correctly rounded
 // division and square root: the surrogate's colour is approximate by design,
@@ -202,9 +203,9 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
 #else
   const ShadeArgs& a = a_by_value;
 #endif
-  __shared__ uint32_t s_pix[kShadePix];
-  __shared__ uint32_t s_face[kShadePix];
-  __shared__ uint32_t s_fb[kShadePix];  // the chunk's pixels for the analytic kernel
+  __shared__ uint32_t s_pix[kNnShadePix];
+  __shared__ uint32_t s_face[kNnShadePix];
+  __shared__ uint32_t s_fb[kNnShadePix];  // the chunk's pixels for the analytic kernel
   __shared__ uint32_t s_n, s_fbn, s_fbbase;
   __shared__ __attribute__((aligned(16))) __bf16 s_x[kBlock / 64][64][16];
   __shared__ __attribute__((aligned(16))) float s_o[kBlock / 64][64][4];
@@ -213,14 +214,14 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
   const int tid = threadIdx.x;
   const int lane = __lane_id(), wv = tid / 64, g = lane / 16, j = lane % 16;
   const long npix = (long)a.cam.W * a.cam.H;
-  const long nchunks = (npix + kShadePix - 1) / kShadePix;
+  const long nchunks = (npix + kNnShadePix - 1) / kNnShadePix;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   nn_stage(pk, s_w, s_b);
   const tina_material& m = a.mat;
   const int fres = m.ops[2].b;  // FresnelFactor slot: metallic = its first operand
   const V3 L = light_at(a.lights, 0).xyz;
   for (long chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    const long base = chunk * kShadePix;
+    const long base = chunk * kNnShadePix;
     __syncthreads();  // the previous chunk's lists are consumed
     if (tid == 0) {
       s_n = 0;
@@ -228,7 +229,7 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kShadePix / kBlock; r++) {
+    for (int r = 0; r < kNnShadePix / kBlock; r++) {
       const long P = base + r * kBlock + tid;
       bool take = false;
       uint32_t f = 0;

@@ -2,7 +2,7 @@
 //
 //   k_shade<PROG, IBL>  render_color + Shader + Lighting + material (+ ACES, + bg)
 //                       (tina/core/triangle.py:133-153, shader.py:112-131,
-//                       lighting.py:84-98): each workgroup owns kShadePix pixels;
+//                       lighting.py:84-98): each workgroup owns a chunk of pixels;
 //                       it reads their keys, paints the empty ones with the
 //                       background, compacts the covered ones into an LDS list
 //                       with a wave ballot, then shades that list densely.
@@ -15,7 +15,15 @@
 
 namespace tina {
 
-constexpr int kShadePix = 1024;  // pixels per shade workgroup
+// Pixels per chunk of a shade workgroup.  The covered pixels of a chunk are
+// shaded kBlock at a time, so the last pass of each chunk runs part-full; one
+// workgroup per chunk (k_shade) also ends with a partial round of workgroups.
+// Measured at 1080p/2M: 512-pixel chunks shade in 62 us against 70 us for 1024
+// (the smaller last round wins); the persistent batch kernel has no last round
+// and prefers 2048 (1.30 ms per 64-view batch against 1.37 ms for 1024).
+constexpr int kShadePix = 512;        // k_shade
+constexpr int kBatchShadePix = 2048;  // k_bshade
+constexpr int kNnShadePix = 1024;     // k_shade_nn and its fallback queue
 
 __device__ __forceinline__ void store_rgb(float* __restrict__ img, long P, V3 c) {
   img[3 * P] = c.x;
@@ -23,23 +31,24 @@ __device__ __forceinline__ void store_rgb(float* __restrict__ img, long P, V3 c)
   img[3 * P + 2] = c.z;
 }
 
-// Pixels [chunk * kShadePix, (chunk + 1) * kShadePix) of one shade call (the body
-// of k_shade).
-template <int PROG, bool IBL>
+// Pixels [chunk * PIX, (chunk + 1) * PIX) of one shade call (the body of k_shade
+// and k_bshade).
+template <int PROG, bool IBL, int PIX>
 __device__ __forceinline__ void shade_chunk(const ShadeArgs& a, const uint64_t* __restrict__ keys,
                                             float* __restrict__ img, long chunk) {
-  __shared__ uint32_t s_pix[kShadePix];
-  __shared__ uint32_t s_face[kShadePix];
+  static_assert(PIX % kBlock == 0, "whole rows of the workgroup");
+  __shared__ uint32_t s_pix[PIX];
+  __shared__ uint32_t s_face[PIX];
   __shared__ uint32_t s_n;
   const int tid = threadIdx.x;
   const long npix = (long)a.cam.W * a.cam.H;
-  const long base = chunk * kShadePix;
+  const long base = chunk * PIX;
   if (tid == 0) s_n = 0;
   __syncthreads();
   const int lane = __lane_id();
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int r = 0; r < kShadePix / kBlock; r++) {
+  for (int r = 0; r < PIX / kBlock; r++) {
     const long P = base + r * kBlock + tid;
     bool take = false;
     uint32_t f = 0;
@@ -84,7 +93,7 @@ k_shade(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __
 #else
   const ShadeArgs& a = a_by_value;  // the host pass only needs the signature
 #endif
-  shade_chunk<PROG, IBL>(a, keys, img, blockIdx.x);
+  shade_chunk<PROG, IBL, kShadePix>(a, keys, img, blockIdx.x);
 }
 
 // One scene of a batched shade: its arguments, key buffer and image.
@@ -107,7 +116,7 @@ k_bshade(const BatchShade* __restrict__ scenes, const int* __restrict__ prefix, 
       if (prefix[mid] <= item) lo = mid; else hi = mid;
     }
     const BatchShade& sc = scenes[lo];
-    shade_chunk<PROG, IBL>(sc.a, sc.keys, sc.img, item - prefix[lo]);
+    shade_chunk<PROG, IBL, kBatchShadePix>(sc.a, sc.keys, sc.img, item - prefix[lo]);
     __syncthreads();
   }
 }

@@ -269,8 +269,8 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
     delete c;
   };
   // segment k takes the fallback pixels of chunks k, k + kFbSeg, ...
-  const long nchunks = ((long)width * height + kShadePix - 1) / kShadePix;
-  c->fb_segcap = (nchunks + kFbSeg - 1) / kFbSeg * kShadePix;
+  const long nchunks = ((long)width * height + kNnShadePix - 1) / kNnShadePix;
+  c->fb_segcap = (nchunks + kFbSeg - 1) / kFbSeg * kNnShadePix;
   if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
       hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, sizeof(BinCounters)) != hipSuccess ||
       hipMalloc(&c->fbq, sizeof(uint32_t) * (size_t)c->fb_segcap * kFbSeg) != hipSuccess ||
@@ -572,7 +572,7 @@ int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, 
   hipStream_t s = (hipStream_t)stream;
   uint32_t* fbn = c->fbn + which * kFbWords;
   uint32_t* fbn_next = c->fbn + (which ^ 1) * kFbWords;
-  hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kShadePix, (unsigned)c->nn_grid)), dim3(kBlock), 0, s, a,
+  hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kNnShadePix, (unsigned)c->nn_grid)), dim3(kBlock), 0, s, a,
                      c->keys, (const __bf16*)d_packed, threshold, c->fbq, c->fb_segcap, fbn, d_img);
   TINA_LAUNCH(c, "k_shade_nn");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade_list<PROG_PBR, true>), dim3(grid_for(npix, kBlock, 512)), dim3(kBlock), 0,
@@ -785,7 +785,7 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     prefix[0 * (nscenes + 1) + i + 1] = prefix[0 * (nscenes + 1) + i] + ngroups;
     prefix[1 * (nscenes + 1) + i + 1] = prefix[1 * (nscenes + 1) + i] + ntiles;
     prefix[2 * (nscenes + 1) + i + 1] =
-        prefix[2 * (nscenes + 1) + i] + (int)(((long)d.width * d.height + kShadePix - 1) / kShadePix);
+        prefix[2 * (nscenes + 1) + i] + (int)(((long)d.width * d.height + kBatchShadePix - 1) / kBatchShadePix);
   }
   for (int k = 0; k < 3; k++) b->totals[k] = prefix[k * (nscenes + 1) + nscenes];
   if (hipMalloc(&b->keys, sizeof(uint64_t) * (size_t)b->total_pix) != hipSuccess ||
