@@ -18,10 +18,10 @@ namespace tina {
 // Pixels per chunk of a shade workgroup.  The covered pixels of a chunk are
 // shaded kBlock at a time, so the last pass of each chunk runs part-full; one
 // workgroup per chunk (k_shade) also ends with a partial round of workgroups.
-// Measured at 1080p/2M: 512-pixel chunks shade in 62 us against 70 us for 1024
-// (the smaller last round wins); the persistent batch kernel has no last round
-// and prefers 2048 (1.30 ms per 64-view batch against 1.37 ms for 1024).
-constexpr int kShadePix = 512;        // k_shade
+// Measured at 1080p/2M: chunks of 256 / 512 / 1024 pixels shade in 57 / 62 /
+// 70 us (the smaller last round wins); the persistent batch kernel has no last
+// round and prefers 2048 (1.29 ms per 64-view batch against 1.37 ms for 1024).
+constexpr int kShadePix = 256;        // k_shade
 constexpr int kBatchShadePix = 2048;  // k_bshade
 constexpr int kNnShadePix = 1024;     // k_shade_nn and its fallback queue
 
