@@ -407,7 +407,10 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       s.ws0 = r2.x; s.ws1 = r2.y; s.ws2 = r2.z; s.Az = r2.w;
       s.Bz = r3.x; s.Cz = r3.y;
       float w0, w1, w2;
-      if (pixel_cover_fast(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1], w0, w1, w2)) {
+      // the reference's weights and test as written: nearly every full wave of
+      // survivors holds a covered pixel, so the divisions run anyway and the
+      // sign-based early rejection of pixel_cover_fast would only add work
+      if (pixel_weights(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1], w0, w1, w2)) {
         const uint64_t key = pack_key(pixel_depth(s, w0, w1, w2), (uint32_t)__float_as_int(r3.w));
         atomicMin(&tk[(ly << kTileLog) + lx], (unsigned long long)key);
       }
