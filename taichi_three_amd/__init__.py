@@ -11,6 +11,7 @@ from .engine import Engine, MAX
 from .field import Field
 from .util.matrix import *
 from .mesh import *
+from .assimp import *
 from .matr import *
 from .lighting import *
 from .shader import *

@@ -15,7 +15,7 @@ import numpy as np
 from .engine import MAX
 
 __all__ = ['SimpleMesh', 'PrimitiveMesh', 'MeshFlatNormal', 'MeshSmoothNormal', 'ConnectiveMesh',
-           'MeshTransform', 'MeshEditBase']
+           'MeshTransform', 'MeshEditBase', 'MeshModel']
 
 
 def _torch():
@@ -296,6 +296,67 @@ class ConnectiveMesh:
         g = (lambda a: None if a is None else a[f].contiguous())
         return dict(verts=g(self.verts_v), norms=g(self.norms_v), coors=g(self.coors_v),
                     colors=g(self.colors_v), attrs=self.attrs, nfaces=self.get_nfaces())
+
+
+class MeshModel:
+    """A mesh read from an OBJ file (tina/mesh/model.py:4-73): vertex, texcoord
+    and normal arrays with faces[nf, 3, 3] (vertex, texcoord, normal) index
+    triples, as tina.readobj returns them.  `obj` is that dictionary or a path
+    (further arguments go to readobj).  The faces are expanded once into a
+    resident [nf, 3, ...] face soup for the raster, on the device."""
+
+    def __init__(self, obj, *args, **kwargs):
+        if isinstance(obj, str):
+            from .assimp import readobj
+            obj = readobj(obj, *args, **kwargs)
+        torch = _torch()
+        f = np.asarray(obj['f'])
+        if f.ndim == 2:  # one index per corner for every stream
+            f = np.stack([f, f, f], axis=2)
+        self.faces = torch.as_tensor(np.ascontiguousarray(f, dtype=np.int64), device='cuda')
+        self.verts_v = _as_device(obj['v'], (3,))
+        self.coors_v = _as_device(obj['vt'], (2,))
+        self.norms_v = _as_device(obj['vn'], (3,))
+        self.maxfaces = int(f.shape[0])
+        self.maxverts = int(self.verts_v.shape[0])
+        self.maxcoors = int(self.coors_v.shape[0])
+        self.maxnorms = int(self.norms_v.shape[0])
+        self.attrs = {}
+        self.version = 0
+        self._soup = None
+
+    def get_npolygon(self):
+        return 3
+
+    def pre_compute(self):
+        pass
+
+    def get_max_vert_nindex(self):
+        return self.maxverts
+
+    def get_nfaces(self):
+        return self.maxfaces
+
+    @property
+    def nfaces(self):
+        return self.maxfaces
+
+    def get_face_vert_indices(self, n):
+        """(i, j, k): the vertex indices of face n."""
+        return tuple(int(x) for x in self.faces[n, :, 0].tolist())
+
+    def set_face_attr(self, name, values):
+        SimpleMesh.set_face_attr(self, name, values)
+
+    def face_buffers(self):
+        if self._soup is None or self._soup[0] != self.version:
+            f = self.faces
+            self._soup = (self.version, dict(verts=self.verts_v[f[:, :, 0]].contiguous(),
+                                             coors=self.coors_v[f[:, :, 1]].contiguous(),
+                                             norms=self.norms_v[f[:, :, 2]].contiguous()))
+        soup = self._soup[1]
+        return dict(verts=soup['verts'], norms=soup['norms'], coors=soup['coors'], colors=None,
+                    attrs=self.attrs, nfaces=self.maxfaces)
 
 
 class MeshSmoothNormal(MeshEditBase):
