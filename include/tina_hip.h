@@ -242,6 +242,20 @@ int tina_mlp_train_step(const float *d_x, const float *d_y, int64_t first, int64
                         float *d_params, float *d_m1, float *d_m2, float *d_partial, float lr, int step,
                         float *d_grad, void *stream);
 
+/* Blooming.apply (tina/postp/blooming.py:45-68) on d_img [height][width][3]:
+ * the bright pass factor * (1 - 1 / (1 + scale * max(0, c - thresh))) averaged
+ * over 2x2 blocks into d_half, blurred along x into d_tmp and along y back into
+ * d_half with the radius + 1 weights d_gwei (taps clamped at the borders), then
+ * d_img += bilerp(d_half, (x / 2, y / 2)).  d_half, d_tmp: [height/2][width/2][3]. */
+int tina_bloom(float *d_img, int width, int height, float thresh, float factor, float scale,
+               const float *d_gwei, int radius, float *d_half, float *d_tmp, void *stream);
+
+/* FXAA.apply (tina/postp/fxaa.py:26-68) on d_img [height][width][3] in place;
+ * d_lumi [height][width] and d_copy [height][width][3] are scratch.  Neighbours
+ * past the border clamp to it. */
+int tina_fxaa(float *d_img, int width, int height, float abs_thresh, float rel_thresh, float factor,
+              float *d_lumi, float *d_copy, void *stream);
+
 /* The analytic PBR + IBL + light-0 shader (ACES) on feature rows d_x [n][12] ->
  * d_out [n][4]: the evaluation the network replaces, on the network's inputs
  * (V is recovered as 2 (n.R) n - R, roughness as exp(x7)). */

@@ -1,0 +1,170 @@
+// postp_kernels.h -- image post-processing on the [H][W][3] f32 image.
+//
+//   k_bloom_down / k_bloom_blur / k_bloom_add   Blooming.apply
+//                       (tina/postp/blooming.py:45-68): bright pass averaged
+//                       over 2x2 blocks into a half-resolution image, a
+//                       separable Gaussian blur (clamped at the borders), and
+//                       the bilinearly upsampled result added to the image.
+//   k_fxaa_lumi / k_fxaa   FXAA.apply (tina/postp/fxaa.py:26-68): luminance and
+//                       a copy of the image, then per pixel the contrast test,
+//                       the edge direction and a bilinear re-sample along it.
+//
+// Every expression keeps the reference's grouping (the library is compiled
+// with -ffp-contract=off).  Neighbour reads past the image clamp to the edge;
+// the reference reads outside its fields there (undefined), so the one-pixel
+// border of FXAA and the last row/column of the bloom upsample are the places
+// this differs from it.  The image is [H][W][3] row-major with y = 0 at the
+// bottom, so the reference's [x, y] is img[(y * W + x) * 3 + c].
+#pragma once
+#include "shade_dev.h"
+
+namespace tina {
+
+__device__ __forceinline__ V3 ld_px(const float* __restrict__ img, int W, int x, int y) {
+  const float* p = img + ((long)y * W + x) * 3;
+  return v3(p[0], p[1], p[2]);
+}
+
+// bilerp (tina/common.py:140-149) over a [h][w][3] image with clamped texels.
+__device__ __forceinline__ V3 bilerp_img(const float* __restrict__ img, int w, int h, float px, float py) {
+  const int ix = f2i_sat(floorf(px)), iy = f2i_sat(floorf(py));
+  const float x0 = px - (float)ix, x1 = py - (float)iy;
+  const float y0 = 1.0f - x0, y1 = 1.0f - x1;
+  const int xa = min(max(ix, 0), w - 1), xb = min(max(ix + 1, 0), w - 1);
+  const int ya = min(max(iy, 0), h - 1), yb = min(max(iy + 1, 0), h - 1);
+  const V3 f00 = ld_px(img, w, xa, ya), f01 = ld_px(img, w, xa, yb);
+  const V3 f10 = ld_px(img, w, xb, ya), f11 = ld_px(img, w, xb, yb);
+  V3 r = vsmul(x1, vsmul(x0, f11));
+  r = vadd(r, vsmul(y1, vsmul(x0, f10)));
+  r = vadd(r, vsmul(y1, vsmul(y0, f00)));
+  r = vadd(r, vsmul(x1, vsmul(y0, f01)));
+  return r;
+}
+
+// Blooming.filter: factor * (1 - 1 / (1 + scale * max(0, x - thresh))), per channel
+__device__ __forceinline__ float bloom_filter(float x, float thresh, float scale, float factor) {
+  float t = fmaxf(0.0f, x - thresh);
+  t = 1.0f - 1.0f / (1.0f + scale * t);
+  return factor * t;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_bloom_down(const float* __restrict__ img, int W, float* __restrict__ half, int hw, int hh, float thresh,
+             float scale, float factor) {
+  const long n = (long)hw * hh;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const int x = (int)(i % hw), y = (int)(i / hw);
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    // ti.ndrange(2, 2): (0, 0), (0, 1), (1, 0), (1, 1)
+#pragma unroll
+    for (int jx = 0; jx < 2; jx++)
+#pragma unroll
+      for (int jy = 0; jy < 2; jy++) {
+        const float* p = img + ((long)(2 * y + jy) * W + 2 * x + jx) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[c] = r[c] + bloom_filter(p[c], thresh, scale, factor);
+      }
+    float* o = half + i * 3;
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[c] = r[c] / 4.0f;
+  }
+}
+
+// One pass of the separable blur: along x (vert = 0) or y (vert = 1), taps
+// clamped to the image: out = in * g0 + sum_i (in[-i] + in[+i]) * g_i.
+__global__ void __launch_bounds__(kBlock)
+k_bloom_blur(const float* __restrict__ src, float* __restrict__ dst, int hw, int hh, const float* __restrict__ gwei,
+             int radius, int vert) {
+  const long n = (long)hw * hh;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const int x = (int)(i % hw), y = (int)(i / hw);
+    const float g0 = gwei[0];
+    const V3 c0 = ld_px(src, hw, x, y);
+    V3 res = v3(c0.x * g0, c0.y * g0, c0.z * g0);
+    for (int k = 1; k <= radius; k++) {
+      V3 val;
+      if (vert) {
+        val = ld_px(src, hw, x, max(0, y - k));
+        val = vadd(val, ld_px(src, hw, x, min(hh - 1, y + k)));
+      } else {
+        val = ld_px(src, hw, max(0, x - k), y);
+        val = vadd(val, ld_px(src, hw, min(hw - 1, x + k), y));
+      }
+      const float g = gwei[k];
+      res = vadd(res, v3(val.x * g, val.y * g, val.z * g));
+    }
+    float* o = dst + i * 3;
+    o[0] = res.x;
+    o[1] = res.y;
+    o[2] = res.z;
+  }
+}
+
+// image[I] += bilerp(half, I / 2)
+__global__ void __launch_bounds__(kBlock)
+k_bloom_add(float* __restrict__ img, int W, int H, const float* __restrict__ half, int hw, int hh) {
+  const long n = (long)W * H;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const int x = (int)(i % W), y = (int)(i / W);
+    const V3 b = bilerp_img(half, hw, hh, (float)x / 2.0f, (float)y / 2.0f);
+    float* p = img + i * 3;
+    p[0] = p[0] + b.x;
+    p[1] = p[1] + b.y;
+    p[2] = p[2] + b.z;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fxaa_lumi(const float* __restrict__ img, float* __restrict__ lumi, float* __restrict__ copy, long n) {
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    const float r = img[3 * i], g = img[3 * i + 1], b = img[3 * i + 2];
+    const float l = ((0.2989f * r) + (0.587f * g)) + (0.114f * b);
+    lumi[i] = fminf(1.0f, fmaxf(0.0f, l));
+    copy[3 * i] = r;
+    copy[3 * i + 1] = g;
+    copy[3 * i + 2] = b;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fxaa(float* __restrict__ img, const float* __restrict__ lumi, const float* __restrict__ copy, int W, int H,
+       float abs_thresh, float rel_thresh, float factor) {
+  const long npix = (long)W * H;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < npix; i += (long)gridDim.x * kBlock) {
+    const int x = (int)(i % W), y = (int)(i / W);
+    auto L = [&](int dx, int dy) {
+      return lumi[(long)min(max(y + dy, 0), H - 1) * W + min(max(x + dx, 0), W - 1)];
+    };
+    const float m = L(0, 0), n = L(0, 1), e = L(1, 0), s = L(0, -1), w = L(-1, 0);
+    const float ne = L(1, 1), nw = L(-1, 1), se = L(1, -1), sw = L(-1, -1);
+    const float hi = fmaxf(fmaxf(fmaxf(fmaxf(m, n), e), s), w);
+    const float lo = fminf(fminf(fminf(fminf(m, n), e), s), w);
+    const float c = hi - lo;
+    if (c < abs_thresh || c < rel_thresh * hi) continue;
+    float filt = 2.0f * (((n + e) + s) + w);
+    filt = filt + (((ne + nw) + se) + sw);
+    filt = fabsf(filt / 12.0f - m);
+    filt = fminf(1.0f, fmaxf(0.0f, filt / c));
+    // smoothstep(filt, 0, 1) ** 2 * factor
+    const float t = fminf(1.0f, fmaxf(0.0f, (filt - 0.0f) / (1.0f - 0.0f)));
+    const float sm = (t * t) * (3.0f - 2.0f * t);
+    float blend = (sm * sm) * factor;
+    float hori = fabsf((n + s) - 2.0f * m) * 2.0f;
+    hori = hori + fabsf((ne + se) - 2.0f * e);
+    hori = hori + fabsf((nw + sw) - 2.0f * w);
+    float vert = fabsf((e + w) - 2.0f * m) * 2.0f;
+    vert = vert + fabsf((ne + nw) - 2.0f * n);
+    vert = vert + fabsf((se + sw) - 2.0f * s);
+    const bool is_hori = hori >= vert;
+    const float plumi = is_hori ? n : e, nlumi = is_hori ? s : w;
+    if (fabsf(plumi - m) < fabsf(nlumi - m)) blend = -blend;
+    const float px = (float)x + blend * (is_hori ? 0.0f : 1.0f);
+    const float py = (float)y + blend * (is_hori ? 1.0f : 0.0f);
+    const V3 r = bilerp_img(copy, W, H, px, py);
+    img[3 * i] = r.x;
+    img[3 * i + 1] = r.y;
+    img[3 * i + 2] = r.z;
+  }
+}
+
+}  // namespace tina

@@ -20,6 +20,7 @@
 #include "nn_kernels.h"
 #include "nn_shade.h"
 #include "shade_kernels.h"
+#include "postp_kernels.h"
 
 using namespace tina;
 
@@ -431,6 +432,37 @@ int tina_tonemap_aces(float* d_img, int npixels, void* stream) {
   const long n = 3L * npixels;
   hipLaunchKernelGGL(k_aces, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_img, n);
   return hip_check(nullptr, hipGetLastError(), "k_aces");
+}
+
+int tina_bloom(float* d_img, int width, int height, float thresh, float factor, float scale, const float* d_gwei,
+               int radius, float* d_half, float* d_tmp, void* stream) {
+  if (!d_img || !d_gwei || !d_half || !d_tmp) return fail(nullptr, TINA_ERR_INVALID, "tina_bloom: buffer is NULL");
+  if (width < 2 || height < 2 || radius < 0)
+    return fail(nullptr, TINA_ERR_INVALID, "tina_bloom: the image needs 2x2 pixels and radius >= 0");
+  const int hw = width / 2, hh = height / 2;
+  hipStream_t s = (hipStream_t)stream;
+  const long nh = (long)hw * hh, n = (long)width * height;
+  hipLaunchKernelGGL(k_bloom_down, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_img, width, d_half, hw, hh,
+                     thresh, scale, factor);
+  hipLaunchKernelGGL(k_bloom_blur, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_half, d_tmp, hw, hh,
+                     d_gwei, radius, 0);
+  hipLaunchKernelGGL(k_bloom_blur, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_tmp, d_half, hw, hh,
+                     d_gwei, radius, 1);
+  hipLaunchKernelGGL(k_bloom_add, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, s, d_img, width, height, d_half,
+                     hw, hh);
+  return hip_check(nullptr, hipGetLastError(), "k_bloom");
+}
+
+int tina_fxaa(float* d_img, int width, int height, float abs_thresh, float rel_thresh, float factor, float* d_lumi,
+              float* d_copy, void* stream) {
+  if (!d_img || !d_lumi || !d_copy) return fail(nullptr, TINA_ERR_INVALID, "tina_fxaa: buffer is NULL");
+  if (width < 1 || height < 1) return fail(nullptr, TINA_ERR_INVALID, "tina_fxaa: empty image");
+  const long n = (long)width * height;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fxaa_lumi, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, s, d_img, d_lumi, d_copy, n);
+  hipLaunchKernelGGL(k_fxaa, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, s, d_img, d_lumi, d_copy, width,
+                     height, abs_thresh, rel_thresh, factor);
+  return hip_check(nullptr, hipGetLastError(), "k_fxaa");
 }
 
 int tina_bake_ibl(const float* d_env, int ew, int eh, float* d_out, int ow, int oh, float roughness, int nsamples,

@@ -14,13 +14,13 @@ from .engine import Engine, MAX
 from .field import Field
 from .lighting import Lighting, SkyboxLighting
 from .matr import Diffuse
-from .postp import ToneMapping
+from .postp import ToneMapping, Blooming, FXAA
 from .raster import TriangleRaster
 from .shader import Shader, ShaderGroup
 
 __all__ = ['Scene']
 
-_UNSUPPORTED = ('taa', 'ssr', 'ssao', 'fxaa', 'blooming')
+_UNSUPPORTED = ('taa', 'ssr', 'ssao')
 
 
 class _Obj:
@@ -67,6 +67,12 @@ class Scene:
         self.objects = {}
         if self.tonemap:
             self.tonemap = ToneMapping(self.res)
+        # post passes in the reference's order (raster.py:200-205): blooming on the
+        # linear image, then tone mapping, then FXAA
+        self.blooming = Blooming(self.res) if options.get('blooming', False) else None
+        self.fxaa = FXAA(self.res) if options.get('fxaa', False) else None
+        if self.blooming is not None and self.neural is not None:
+            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming')
 
     def _ensure_material_shader(self, material):
         if material in self.materials:
@@ -122,7 +128,19 @@ class Scene:
         mark('end')
 
     def _shade_all(self, order):
-        tm = _abi.TONEMAP if self.tonemap else 0
+        # tone mapping is fused into the shade kernels unless blooming has to see the linear image
+        tm = _abi.TONEMAP if self.tonemap and self.blooming is None else 0
+        if self.blooming is not None and self.neural is not None:
+            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming')
+        self._shade_objects(order, tm)
+        if self.blooming is not None:
+            self.blooming.apply(self.image)
+            if self.tonemap:
+                self.tonemap.apply(self.image)
+        if self.fxaa is not None:
+            self.fxaa.apply(self.image)
+
+    def _shade_objects(self, order, tm):
         if not order:
             self._fill_background(tm)
             return
