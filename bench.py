@@ -170,6 +170,37 @@ def neural_vs_analytic(reps, train_steps):
             'mlp_flop_per_px': 2 * (12 * 64 + 64 * 64 * 2 + 64 * 4)}
 
 
+def post_passes(reps):
+    """Blooming and FXAA on the 1920x1080 image (device ms per pass, median)
+    with the algorithmic bytes each moves: bloom reads the image and adds to it
+    (3 x 12 B/px) plus its half-resolution passes (2 x 2 x 12 B per half pixel);
+    FXAA reads the image, writes luminance and a copy, reads them back and
+    writes the image (12 + 4 + 12 + 4 + 12 + 12 B/px)."""
+    import numpy as np
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd.field import Field
+    W, H = 1920, 1080
+    img = Field((torch.rand((H, W, 3), device='cuda') * 3).contiguous())
+    out = {}
+    for name, p, nbytes in (('bloom', tina.Blooming((W, H)), 36 * W * H + 48 * (W // 2) * (H // 2)),
+                            ('fxaa', tina.FXAA((W, H)), 56 * W * H)):
+        p.apply(img)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            p.apply(img)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = float(np.median(ts))
+        out[name] = {'ms': round(ms, 4), 'algorithmic_GBs': round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    out['bloom']['radius'] = min(W, H) // 16
+    return out
+
+
 def batched_views(reps, tune_reps=3):
     """64 independent scenes (1k-500k faces log-uniform, resolutions from
     {256^2, 512^2, 1024^2, 1920x1080}, cameras on the radius-3 sphere, seed 2)
@@ -311,6 +342,7 @@ def main():
         out['neural'] = neural_vs_analytic(max(10, args.steps // 2), args.train_steps)
     if not args.no_batch:
         out['batch'] = batched_views(max(5, args.steps // 4))
+    out['postp'] = post_passes(max(10, args.steps // 2))
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(w)
         out['speedup_vs_cpu'] = round(fps / out['cpu_baseline']['value'], 1)
