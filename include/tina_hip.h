@@ -256,6 +256,11 @@ int tina_bloom(float *d_img, int width, int height, float thresh, float factor, 
 int tina_fxaa(float *d_img, int width, int height, float abs_thresh, float rel_thresh, float factor,
               float *d_lumi, float *d_copy, void *stream);
 
+/* Accumator.update (tina/util/accumator.py:17-23): with inv = 1 / count (count
+ * counts the frames so far, this one included), d_acc[i] = d_acc[i] * (1 - inv)
+ * + d_src[i] * inv for i < n floats. */
+int tina_accumulate(float *d_acc, const float *d_src, int64_t n, int count, void *stream);
+
 /* The analytic PBR + IBL + light-0 shader (ACES) on feature rows d_x [n][12] ->
  * d_out [n][4]: the evaluation the network replaces, on the network's inputs
  * (V is recovered as 2 (n.R) n - R, roughness as exp(x7)). */

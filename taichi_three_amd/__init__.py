@@ -16,7 +16,7 @@ from .matr import *
 from .lighting import *
 from .shader import *
 from .raster import TriangleRaster
-from .postp import ToneMapping, Blooming, FXAA
+from .postp import ToneMapping, Blooming, FXAA, Accumator
 from .probe import ProbeShader
 from .scene import Scene
 from . import _abi

@@ -136,6 +136,7 @@ EXPORTS = [
     ('tina_bloom', _i32, [_vp, _i32, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _i32, _vp, _vp,
                           _vp]),
     ('tina_fxaa', _i32, [_vp, _i32, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),
+    ('tina_accumulate', _i32, [_vp, _vp, _i64, _i32, _vp]),
     ('tina_nn_analytic_rows', _i32, [ctypes.POINTER(Lights), ctypes.POINTER(IBLTables), _vp, _i64,
                                      ctypes.c_float, _vp, _vp]),
     ('tina_mlp_inference', _i32, [_vp, _vp, _i64, _vp, _vp]),

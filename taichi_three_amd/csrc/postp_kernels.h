@@ -8,6 +8,8 @@
 //   k_fxaa_lumi / k_fxaa   FXAA.apply (tina/postp/fxaa.py:26-68): luminance and
 //                       a copy of the image, then per pixel the contrast test,
 //                       the edge direction and a bilinear re-sample along it.
+//   k_accumulate        Accumator.update (tina/util/accumator.py:17-23): the
+//                       running mean of the frames of a jittered (TAA) sequence.
 //
 // Every expression keeps the reference's grouping (the library is compiled
 // with -ffp-contract=off).  Neighbour reads past the image clamp to the edge;
@@ -164,6 +166,17 @@ k_fxaa(float* __restrict__ img, const float* __restrict__ lumi, const float* __r
     img[3 * i] = r.x;
     img[3 * i + 1] = r.y;
     img[3 * i + 2] = r.z;
+  }
+}
+
+// Accumator.update (tina/util/accumator.py:17-23): acc = acc * (1 - inv) + src * inv
+__global__ void __launch_bounds__(kBlock)
+k_accumulate(float* __restrict__ acc, const float* __restrict__ src, long n, float inv) {
+  const float keep = 1.0f - inv;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+    float a = acc[i] * keep;
+    a = a + src[i] * inv;
+    acc[i] = a;
   }
 }
 

@@ -465,6 +465,16 @@ int tina_fxaa(float* d_img, int width, int height, float abs_thresh, float rel_t
   return hip_check(nullptr, hipGetLastError(), "k_fxaa");
 }
 
+int tina_accumulate(float* d_acc, const float* d_src, int64_t n, int count, void* stream) {
+  if (!d_acc || !d_src || n < 0) return fail(nullptr, TINA_ERR_INVALID, "tina_accumulate: bad buffer or size");
+  if (count < 1) return fail(nullptr, TINA_ERR_INVALID, "tina_accumulate: count counts from 1");
+  if (n == 0) return TINA_OK;
+  const float inv = 1.0f / (float)count;
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_acc, d_src,
+                     (long)n, inv);
+  return hip_check(nullptr, hipGetLastError(), "k_accumulate");
+}
+
 int tina_bake_ibl(const float* d_env, int ew, int eh, float* d_out, int ow, int oh, float roughness, int nsamples,
                   void* stream) {
   if (!d_env || !d_out) return fail(nullptr, TINA_ERR_INVALID, "tina_bake_ibl: map pointer is NULL");

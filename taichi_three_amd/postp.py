@@ -1,11 +1,12 @@
 """Image post-processing on the [H, W, 3] f32 image: ToneMapping
-(tina/postp/tonemap.py:4-12, in-place ACES), Blooming (tina/postp/blooming.py)
-and FXAA (tina/postp/fxaa.py), each one call into libtina_hip.so."""
+(tina/postp/tonemap.py:4-12, in-place ACES), Blooming (tina/postp/blooming.py),
+FXAA (tina/postp/fxaa.py) and the TAA Accumator (tina/util/accumator.py), each
+one call into libtina_hip.so."""
 import ctypes
 
 from . import _abi
 
-__all__ = ['ToneMapping', 'Blooming', 'FXAA']
+__all__ = ['ToneMapping', 'Blooming', 'FXAA', 'Accumator']
 
 
 class ToneMapping:
@@ -88,3 +89,29 @@ class FXAA:
         _abi.check(_abi.load().tina_fxaa(_abi.ptr(t), W, H, ctypes.c_float(self.abs_thresh),
                                          ctypes.c_float(self.rel_thresh), ctypes.c_float(self.factor),
                                          _abi.ptr(self.lumi), _abi.ptr(self.img), _abi.stream_handle()))
+
+
+class Accumator:
+    """Running mean of a sequence of frames (tina/util/accumator.py:4-23): the
+    temporal anti-aliasing buffer of Scene(taa=True)."""
+
+    def __init__(self, res):
+        import torch
+        from .field import Field
+        W, H = int(res[0]), int(res[1])
+        self._t = torch.zeros((H, W, 3), dtype=torch.float32,
+                              device=torch.device('cuda', torch.cuda.current_device()))
+        self.img = Field(self._t)
+        self.count = 0
+
+    def clear(self):
+        self.count = 0
+        self._t.zero_()
+
+    def update(self, src):
+        t, W, H = _dims(src)
+        if tuple(t.shape) != tuple(self._t.shape):
+            raise ValueError('Accumator.update: image size differs')
+        self.count += 1
+        _abi.check(_abi.load().tina_accumulate(_abi.ptr(self._t), _abi.ptr(t), int(t.numel()), int(self.count),
+                                               _abi.stream_handle()))

@@ -14,13 +14,13 @@ from .engine import Engine, MAX
 from .field import Field
 from .lighting import Lighting, SkyboxLighting
 from .matr import Diffuse
-from .postp import ToneMapping, Blooming, FXAA
+from .postp import ToneMapping, Blooming, FXAA, Accumator
 from .raster import TriangleRaster
 from .shader import Shader, ShaderGroup
 
 __all__ = ['Scene']
 
-_UNSUPPORTED = ('taa', 'ssr', 'ssao')
+_UNSUPPORTED = ('ssr', 'ssao')
 
 
 class _Obj:
@@ -71,6 +71,9 @@ class Scene:
         # linear image, then tone mapping, then FXAA
         self.blooming = Blooming(self.res) if options.get('blooming', False) else None
         self.fxaa = FXAA(self.res) if options.get('fxaa', False) else None
+        # temporal anti-aliasing: a jittered pixel centre per frame, averaged (raster.py:83-84, 173-174, 206-214)
+        self.taa = bool(options.get('taa', False))
+        self.accum = Accumator(self.res) if self.taa else None
         if self.blooming is not None and self.neural is not None:
             raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming')
 
@@ -111,6 +114,8 @@ class Scene:
         'shade' stages and once more with 'end' (used to place timing events)."""
         eng = self.engine
         mark = on_phase or (lambda name: None)
+        if self.taa:
+            eng.randomize_bias(self.accum.count == 0)
         for s in self.pre_shaders + self.post_shaders:
             s.clear_buffer()
         mark('clear')
@@ -125,6 +130,8 @@ class Scene:
             order.append((obj, info))
         mark('shade')
         self._shade_all(order)
+        if self.taa:
+            self.accum.update(self.image)
         mark('end')
 
     def _shade_all(self, order):
@@ -169,11 +176,14 @@ class Scene:
 
     @property
     def img(self):
-        """The final (tone-mapped) image, a Field of shape (W, H, 3)."""
-        return self.image
+        """The final (tone-mapped) image, a Field of shape (W, H, 3); with taa the
+        running mean of the frames since the last clear()."""
+        return self.accum.img if self.taa else self.image
 
     def clear(self):
-        pass
+        """Restarts the TAA average (raster.py:230-232)."""
+        if self.taa:
+            self.accum.clear()
 
     def _fast_export_image(self, out):
         """Flat RGB export, out[(y * W + x) * 3 + c] (raster.py:243-250)."""

@@ -58,3 +58,39 @@ def test_scene_post_chain(gpu):
     lin, post = imgs
     want = postp.fxaa(orc.aces(postp.bloom(lin, gwei=bl.gwei.cpu().numpy(), radius=bl.radius)))
     assert np.abs(post - want).max() <= 1e-5
+
+
+def test_taa_is_the_running_mean_of_the_jittered_frames(gpu):
+    """Scene(taa=True): frame k is rendered with a random pixel-centre bias (the
+    first with the centre), and img is the running mean acc = acc (1 - 1/k) +
+    frame / k.  Rendering the same biases without TAA and averaging the same way
+    in binary32 gives the same image; clear() restarts the mean."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import workloads as wl
+    print('case: taa running mean', flush=True)
+    w = wl.plain_512(seed=5)
+    taa, _ = wl.build_scene(w, taa=True)
+    plain, _ = wl.build_scene(w)
+    np.random.seed(3)
+    acc = None
+    biases = []
+    for k in range(1, 5):
+        taa.render()
+        biases.append(taa.engine.bias.copy())
+        plain.engine.bias = biases[-1]
+        plain.engine._push_camera()
+        plain.render()
+        torch.cuda.synchronize()
+        frame = plain.img.to_numpy()
+        inv = f32(1.0) / f32(k)
+        acc = frame.copy() * f32(0) if acc is None else acc
+        acc = (acc * (f32(1) - inv)).astype(f32)
+        acc = (acc + frame * inv).astype(f32)
+    assert np.array_equal(biases[0], np.array([0.5, 0.5], f32)) and not np.array_equal(biases[1], biases[0])
+    got = taa.img.to_numpy()
+    assert np.abs(got - acc).max() <= 1e-6
+    taa.clear()
+    taa.render()
+    torch.cuda.synchronize()
+    assert np.array_equal(taa.engine.bias, np.array([0.5, 0.5], f32))
