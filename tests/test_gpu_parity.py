@@ -342,3 +342,36 @@ def test_errors_raise(gpu):
     from taichi_three_amd import _abi
     with pytest.raises(ValueError):
         _abi.check(eng._lib.tina_raster(eng._ctx, _abi.ptr(m.verts), 20, 0, 3, None), eng._ctx)
+
+
+@pytest.mark.parametrize('seed', list(range(8)))
+def test_raster_random_scenes(gpu, seed):
+    """Random soups, cameras and screens (binned path: bbox shrinking, giant
+    faces, strips of tiles) against the oracle, bit for bit, culling on and off."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import workloads as wl
+    rng = np.random.default_rng(1000 + seed)
+    W, H = int(rng.integers(64, 1400)), int(rng.integers(64, 900))
+    n = int(rng.integers(2000, 60000))
+    verts = wl.triangle_soup(rng, n, np.log(rng.uniform(0.005, 0.2)), rng.uniform(0.3, 1.2))
+    d = rng.normal(size=3)
+    view = tina.lookat(pos=tuple(rng.uniform(-0.3, 0.3, 3)), back=tuple(rng.uniform(1.5, 4.0) * d / np.linalg.norm(d)))
+    proj = tina.perspective(float(rng.uniform(30, 100)), W / H, 0.05, 500)
+    W2V, V2W = tina.camera_matrices(view, proj)
+    cam = orc.camera(W2V, V2W)
+    for culling in (True, False):
+        print(f'case: random scene seed={seed} {W}x{H} n={n} culling={culling}', flush=True)
+        eng = tina.Engine(W, H, maxfaces=n)
+        eng.set_camera(view, proj)
+        r = tina.TriangleRaster(eng, maxfaces=n, culling=culling)
+        m = tina.SimpleMesh(maxfaces=n)
+        m.set_face_verts(verts)
+        eng.clear_depth()
+        r.set_object(m)
+        r.render_occup()
+        ids, depth = eng.resolve()
+        torch.cuda.synchronize()
+        od, oi = orc.raster(W, H, cam, verts, flags=(1 if culling else 0) | 2)
+        assert np.array_equal(ids.cpu().numpy(), oi), f'{(ids.cpu().numpy() != oi).sum()} id mismatches'
+        assert np.array_equal(depth.cpu().numpy(), od)
