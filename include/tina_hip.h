@@ -46,6 +46,9 @@ extern "C" {
 #define TINA_BG      16u    /* write the background colour where no face won */
 #define TINA_IBL     32u    /* SkyboxLighting instead of Lighting (tina/core/lighting.py:4-21) */
 #define TINA_NOLIGHT 64u    /* skip direct lights (IBL only) */
+#define TINA_PARTICLES 128u /* the mesh holds particles: verts = centres [n][3], norms = radii [n],
+                               colors = per-particle colours [n][3] (ParticleRaster.render_color,
+                               tina/core/particle.py:119-148) */
 
 /* ---- limits -------------------------------------------------------------- */
 #define TINA_MAX_LIGHTS  16   /* Lighting(maxlights=16), tina/core/lighting.py:25 */
@@ -178,6 +181,14 @@ int tina_clear(tina_ctx *ctx, void *stream);
  * depth and the depth|id min for faces [0, T) of d_verts with ids id_base + f. */
 int tina_raster(tina_ctx *ctx, const float *d_verts, int nfaces, uint32_t id_base,
                 uint32_t flags, void *stream);
+
+/* ParticleRaster.render_occup (tina/core/particle.py:77-116): the disc of each
+ * particle i in [0, n) (centre d_pos[i] [n][3], radius d_radius[i]) enters the
+ * depth|id min with depth int(z_view * 2**30) and id id_base + i.  flags:
+ * TINA_CLIP drops particles whose view-space z lies outside [-1, 1]
+ * (particle.py:85-87).  Shade the result with tina_shade and TINA_PARTICLES. */
+int tina_raster_particles(tina_ctx *ctx, const float *d_pos, const float *d_radius, int n,
+                          uint32_t id_base, uint32_t flags, void *stream);
 
 /* TriangleRaster.render_color + Shader/Lighting/material (triangle.py:133-153,
  * shader.py:112-131, lighting.py:84-98): shades the pixels whose winning id lies

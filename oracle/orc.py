@@ -56,6 +56,27 @@ def raster(W, H, cam, verts, id_base=0, flags=3, depth=None, occup=None):
     return depth, occup
 
 
+def raster_particles(W, H, cam, pos, radius, id_base=0, flags=2, depth=None, occup=None):
+    """Serial particle coverage (orc_raster_particles).  Returns (depth, occup) [H, W] i32."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+    radius = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, dtype=np.float32), (len(pos),)))
+    if depth is None:
+        depth = np.empty((H, W), dtype=np.int32)
+        occup = np.empty((H, W), dtype=np.int32)
+        lib().orc_clear(W, H, _p(depth), _p(occup))
+    lib().orc_raster_particles(W, H, ctypes.byref(cam), _p(pos), _p(radius), len(pos), ctypes.c_uint32(id_base),
+                               ctypes.c_uint32(flags), _p(depth), _p(occup))
+    return depth, occup
+
+
+def particle_mesh(pos, radius, colors=None, id_base=0):
+    """HostMesh of a particle set for shade(..., flags | PARTICLES): centres in
+    verts, radii in norms, colours in colors."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+    radius = np.broadcast_to(np.asarray(radius, dtype=np.float32), (len(pos),))
+    return HostMesh(pos, norms=radius, colors=colors, id_base=id_base)
+
+
 def raster_omp(W, H, cam, verts, id_base=0, flags=3, keys=None, nthreads=1):
     verts = np.ascontiguousarray(verts, dtype=np.float32)
     if keys is None:

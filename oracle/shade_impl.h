@@ -349,51 +349,73 @@ void FN(orc_shade)(int W, int H, const tina_camera *cam, const tina_mesh *mesh,
         if (fu >= (uint32_t)mesh->nfaces) continue;
         long f = (long)fu;
 
-        /* render_color (triangle.py:135-151): the setup of :93-113 recomputed from
-         * the vertices, then the same weights as the coverage pass. */
-        VEC A = FN(load3)(mesh->verts, 3 * f), B = FN(load3)(mesh->verts, 3 * f + 1),
-            C = FN(load3)(mesh->verts, 3 * f + 2);
-        REAL wA, wB, wC;
-        VEC Bv = FN(mapply_pos)(cam->W2V, B, &wB);
-        VEC Cv = FN(mapply_pos)(cam->W2V, C, &wC);
-        VEC Av = FN(mapply_pos)(cam->W2V, A, &wA);
-        REAL ax = (Av.x * RC(0.5) + RC(0.5)) * resx, ay = (Av.y * RC(0.5) + RC(0.5)) * resy;
-        REAL bx = (Bv.x * RC(0.5) + RC(0.5)) * resx, by = (Bv.y * RC(0.5) + RC(0.5)) * resy;
-        REAL cx = (Cv.x * RC(0.5) + RC(0.5)) * resx, cy = (Cv.y * RC(0.5) + RC(0.5)) * resy;
-        REAL nn = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
-        REAL bcnx = (bx - cx) / nn, bcny = (by - cy) / nn;
-        REAL canx = (cx - ax) / nn, cany = (cy - ay) / nn;
-        REAL ws0 = RC(1) / wA, ws1 = RC(1) / wB, ws2 = RC(1) / wC;
-        REAL w_bc = (p_x - bx) * bcny - (p_y - by) * bcnx;
-        REAL w_ca = (p_x - cx) * cany - (p_y - cy) * canx;
-        REAL t = (RC(1) - w_bc) - w_ca;
-        REAL wx = w_bc * ws0, wy = w_ca * ws1, wz = t * ws2;
-        REAL s = (wx + wy) + wz;
-        wx = wx / s; wy = wy / s; wz = wz / s;
-
-        /* interpolate (triangle.py:31-49) */
-        VEC pos = FN(vadd)(FN(vadd)(FN(vsmul)(wx, A), FN(vsmul)(wy, B)), FN(vsmul)(wz, C));
-        VEC nrm;
-        if ((flags & TINA_SMOOTH) && mesh->norms) {
-            VEC An = FN(load3)(mesh->norms, 3 * f), Bn = FN(load3)(mesh->norms, 3 * f + 1),
-                Cn = FN(load3)(mesh->norms, 3 * f + 2);
-            nrm = FN(vadd)(FN(vadd)(FN(vsmul)(wx, An), FN(vsmul)(wy, Bn)), FN(vsmul)(wz, Cn));
+        VEC pos, nrm, tc, color;
+        if (flags & TINA_PARTICLES) {
+            /* ParticleRaster.render_color (particle.py:125-146): the front-hemisphere
+             * point behind the pixel on the sphere of centre Al, radius Rl */
+            VEC Al = FN(load3)(mesh->verts, f);
+            REAL Rl = (REAL)((const float *)(uintptr_t)mesh->norms)[f];
+            VEC Av = FN(mapply_pos)(cam->W2V, Al, NULL);
+            REAL vx = ((p_x / resx) * RC(2)) - RC(1), vy = ((p_y / resy) * RC(2)) - RC(1);
+            VEC Pl = FN(mapply_pos)(cam->V2W, FN(v3)(vx, vy, Av.z), NULL);
+            VEC Dl = FN(vdivs)(FN(vsub)(Pl, Al), Rl);
+            VEC Zl = FN(v3)((REAL)cam->V2W[3] * RC(0), (REAL)cam->V2W[7] * RC(0), (REAL)cam->V2W[11] * RC(0));
+            Zl = FN(v3)(Zl.x + (REAL)cam->V2W[0] * RC(0), Zl.y + (REAL)cam->V2W[4] * RC(0), Zl.z + (REAL)cam->V2W[8] * RC(0));
+            Zl = FN(v3)(Zl.x + (REAL)cam->V2W[1] * RC(0), Zl.y + (REAL)cam->V2W[5] * RC(0), Zl.z + (REAL)cam->V2W[9] * RC(0));
+            Zl = FN(v3)(Zl.x + (REAL)cam->V2W[2] * RC(1), Zl.y + (REAL)cam->V2W[6] * RC(1), Zl.z + (REAL)cam->V2W[10] * RC(1));
+            Zl = FN(vnormalized)(Zl);                       /* mapply_dir(V2W, (0, 0, 1)).normalized() */
+            Dl = FN(vsub)(Dl, FN(vsmul)(FN(r_sqrt)(RC(1) - FN(vdot)(Dl, Dl)), Zl));
+            Dl = FN(vnormalized)(Dl);
+            nrm = Dl;
+            pos = FN(vadd)(Al, FN(vsmul)(Rl, Dl));
+            tc = FN(vs)(RC(0));
+            color = (flags & TINA_VCOLOR) && mesh->colors ? FN(load3)(mesh->colors, f) : FN(vs)(RC(1));
         } else {
-            nrm = FN(vcross)(FN(vsub)(B, A), FN(vsub)(C, A));
-        }
-        nrm = FN(vnormalized)(nrm);
-        VEC tc = FN(vs)(RC(0));
-        if ((flags & TINA_TEX) && mesh->coors) {
-            const float *q = (const float *)(uintptr_t)mesh->coors + 6 * f;
-            REAL tx = ((wx * (REAL)q[0]) + (wy * (REAL)q[2])) + (wz * (REAL)q[4]);
-            REAL ty = ((wx * (REAL)q[1]) + (wy * (REAL)q[3])) + (wz * (REAL)q[5]);
-            tc = FN(v3)(tx, ty, RC(0));
-        }
-        VEC color = FN(vs)(RC(1));
-        if ((flags & TINA_VCOLOR) && mesh->colors) {
-            VEC Ac = FN(load3)(mesh->colors, 3 * f), Bc = FN(load3)(mesh->colors, 3 * f + 1),
-                Cc = FN(load3)(mesh->colors, 3 * f + 2);
-            color = FN(vadd)(FN(vadd)(FN(vsmul)(wx, Ac), FN(vsmul)(wy, Bc)), FN(vsmul)(wz, Cc));
+            /* render_color (triangle.py:135-151): the setup of :93-113 recomputed from
+             * the vertices, then the same weights as the coverage pass. */
+            VEC A = FN(load3)(mesh->verts, 3 * f), B = FN(load3)(mesh->verts, 3 * f + 1),
+                C = FN(load3)(mesh->verts, 3 * f + 2);
+            REAL wA, wB, wC;
+            VEC Bv = FN(mapply_pos)(cam->W2V, B, &wB);
+            VEC Cv = FN(mapply_pos)(cam->W2V, C, &wC);
+            VEC Av = FN(mapply_pos)(cam->W2V, A, &wA);
+            REAL ax = (Av.x * RC(0.5) + RC(0.5)) * resx, ay = (Av.y * RC(0.5) + RC(0.5)) * resy;
+            REAL bx = (Bv.x * RC(0.5) + RC(0.5)) * resx, by = (Bv.y * RC(0.5) + RC(0.5)) * resy;
+            REAL cx = (Cv.x * RC(0.5) + RC(0.5)) * resx, cy = (Cv.y * RC(0.5) + RC(0.5)) * resy;
+            REAL nn = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+            REAL bcnx = (bx - cx) / nn, bcny = (by - cy) / nn;
+            REAL canx = (cx - ax) / nn, cany = (cy - ay) / nn;
+            REAL ws0 = RC(1) / wA, ws1 = RC(1) / wB, ws2 = RC(1) / wC;
+            REAL w_bc = (p_x - bx) * bcny - (p_y - by) * bcnx;
+            REAL w_ca = (p_x - cx) * cany - (p_y - cy) * canx;
+            REAL t = (RC(1) - w_bc) - w_ca;
+            REAL wx = w_bc * ws0, wy = w_ca * ws1, wz = t * ws2;
+            REAL s = (wx + wy) + wz;
+            wx = wx / s; wy = wy / s; wz = wz / s;
+
+            /* interpolate (triangle.py:31-49) */
+            pos = FN(vadd)(FN(vadd)(FN(vsmul)(wx, A), FN(vsmul)(wy, B)), FN(vsmul)(wz, C));
+            if ((flags & TINA_SMOOTH) && mesh->norms) {
+                VEC An = FN(load3)(mesh->norms, 3 * f), Bn = FN(load3)(mesh->norms, 3 * f + 1),
+                    Cn = FN(load3)(mesh->norms, 3 * f + 2);
+                nrm = FN(vadd)(FN(vadd)(FN(vsmul)(wx, An), FN(vsmul)(wy, Bn)), FN(vsmul)(wz, Cn));
+            } else {
+                nrm = FN(vcross)(FN(vsub)(B, A), FN(vsub)(C, A));
+            }
+            nrm = FN(vnormalized)(nrm);
+            tc = FN(vs)(RC(0));
+            if ((flags & TINA_TEX) && mesh->coors) {
+                const float *q = (const float *)(uintptr_t)mesh->coors + 6 * f;
+                REAL tx = ((wx * (REAL)q[0]) + (wy * (REAL)q[2])) + (wz * (REAL)q[4]);
+                REAL ty = ((wx * (REAL)q[1]) + (wy * (REAL)q[3])) + (wz * (REAL)q[5]);
+                tc = FN(v3)(tx, ty, RC(0));
+            }
+            color = FN(vs)(RC(1));
+            if ((flags & TINA_VCOLOR) && mesh->colors) {
+                VEC Ac = FN(load3)(mesh->colors, 3 * f), Bc = FN(load3)(mesh->colors, 3 * f + 1),
+                    Cc = FN(load3)(mesh->colors, 3 * f + 2);
+                color = FN(vadd)(FN(vadd)(FN(vsmul)(wx, Ac), FN(vsmul)(wy, Bc)), FN(vsmul)(wz, Cc));
+            }
         }
 
         /* calc_viewdir (shader.py:81-93) */

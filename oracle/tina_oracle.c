@@ -165,6 +165,86 @@ void orc_raster(int W, int H, const tina_camera *cam, const float *verts, int T,
     }
 }
 
+/* ---- particles: tina/core/particle.py:77-116 (render_occup) ---------------- */
+
+/* mapply_pos(M, p) = (r / w) per component (tina/common.py:180-183) */
+static inline void mapply_pos3(const float *M, const float p[3], float out[3])
+{
+    float r[3], w;
+    mapply1(M, p, r, &w);
+    for (int i = 0; i < 3; i++) out[i] = r[i] / w;
+}
+
+/* mapply_dir(M, e_axis).normalized() (common.py:186-189; Matrix.normalized):
+ * mapply with weight 0, then v * (1 / sqrt(v.v)) */
+static inline void unit_dir(const float *M, int axis, float out[3])
+{
+    const float e[3] = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+    for (int i = 0; i < 3; i++) {
+        float acc = M[4 * i + 3] * 0.0f;
+        for (int j = 0; j < 3; j++) acc = acc + M[4 * i + j] * e[j];
+        out[i] = acc;
+    }
+    float inv = 1.0f / sqrtf(((out[0] * out[0]) + (out[1] * out[1])) + (out[2] * out[2]));
+    for (int i = 0; i < 3; i++) out[i] = inv * out[i];
+}
+
+void orc_raster_particles(int W, int H, const tina_camera *cam, const float *pos, const float *radius, int n,
+                          uint32_t id_base, uint32_t flags, int32_t *depth, int32_t *occup)
+{
+    const float res[2] = {(float)W, (float)H};
+    float DX[3], DY[3];
+    unit_dir(cam->V2W, 0, DX);                                           /* :89 */
+    unit_dir(cam->V2W, 1, DY);                                           /* :90 */
+    for (int f = 0; f < n; f++) {
+        const float *Al = pos + 3L * f;
+        const float Rl = radius[f];
+        float Av[3];
+        mapply_pos3(cam->W2V, Al, Av);                                   /* :84 */
+        if ((flags & TINA_CLIP) && !(-1.0f <= Av[2] && Av[2] <= 1.0f)) continue;  /* :85-87 */
+        /* Rv = (to_viewspace(Al + DX*Rl).x - Av.x, to_viewspace(Al + DY*Rl).y - Av.y)  (:91-93) */
+        float px_[3], py_[3], tx[3], ty[3];
+        for (int i = 0; i < 3; i++) {
+            px_[i] = Al[i] + DX[i] * Rl;
+            py_[i] = Al[i] + DY[i] * Rl;
+        }
+        mapply_pos3(cam->W2V, px_, tx);
+        mapply_pos3(cam->W2V, py_, ty);
+        const float Rv[2] = {tx[0] - Av[0], ty[1] - Av[1]};
+        /* the four points Av -+ (Rv.x, 0, 0), Av -+ (0, Rv.y, 0) in viewport units (:94-101);
+         * bot = ifloor(min(b0, b2)), top = iceil(max(b1, b3)), clamped (:103-104) */
+        int32_t bot[2], top[2];
+        for (int i = 0; i < 2; i++) {
+            /* along axis i one of the two points is moved by -+Rv[i], the other is not */
+            const float lo_moved = ((Av[i] - Rv[i]) * 0.5f + 0.5f) * res[i];
+            const float lo_still = ((Av[i] - 0.0f) * 0.5f + 0.5f) * res[i];
+            const float hi_moved = ((Av[i] + Rv[i]) * 0.5f + 0.5f) * res[i];
+            const float hi_still = ((Av[i] + 0.0f) * 0.5f + 0.5f) * res[i];
+            const int32_t bo = f2i_sat(floorf(fminf(lo_moved, lo_still)));
+            const int32_t to = f2i_sat(ceilf(fmaxf(hi_moved, hi_still)));
+            bot[i] = bo > 0 ? bo : 0;
+            top[i] = to < (i == 0 ? W - 1 : H - 1) ? to : (i == 0 ? W - 1 : H - 1);
+        }
+        const float r2 = Rl * Rl;
+        const int32_t d = f2i_sat(Av[2] * 1073741824.0f);              /* :112-113 */
+        for (int32_t X = bot[0]; X <= top[0]; X++)
+            for (int32_t Y = bot[1]; Y <= top[1]; Y++) {
+                /* Pl = from_viewspace((P + bias) / res * 2 - 1, Av.z)   (:106-108) */
+                const float Pv[3] = {((((float)X + cam->bias[0]) / res[0]) * 2.0f) - 1.0f,
+                                     ((((float)Y + cam->bias[1]) / res[1]) * 2.0f) - 1.0f, Av[2]};
+                float Pl[3];
+                mapply_pos3(cam->V2W, Pv, Pl);
+                const float e0 = Pl[0] - Al[0], e1 = Pl[1] - Al[1], e2 = Pl[2] - Al[2];
+                if ((((e0 * e0) + (e1 * e1)) + (e2 * e2)) > r2) continue;   /* :109-110 */
+                const long P = (long)Y * W + X;
+                if (d < depth[P]) {          /* serial form of atomic_min + occup (:114-116) */
+                    depth[P] = d;
+                    occup[P] = (int32_t)(id_base + (uint32_t)f);
+                }
+            }
+    }
+}
+
 static inline uint64_t pack_key(int32_t depth, uint32_t id)
 {
     return ((uint64_t)((uint32_t)depth ^ 0x80000000u) << 32) | (uint64_t)id;

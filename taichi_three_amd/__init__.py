@@ -16,6 +16,8 @@ from .matr import *
 from .lighting import *
 from .shader import *
 from .raster import TriangleRaster
+from .pars import *
+from .particle import ParticleRaster
 from .postp import ToneMapping, Blooming, FXAA, Accumator
 from .probe import ProbeShader
 from .scene import Scene

@@ -26,6 +26,7 @@ TONEMAP = 8
 BG = 16
 IBL = 32
 NOLIGHT = 64
+PARTICLES = 128
 
 MAX_LIGHTS = 16
 MAX_PARAMS = 12
@@ -121,6 +122,7 @@ EXPORTS = [
     ('tina_set_camera', _i32, [_vp, ctypes.POINTER(Camera)]),
     ('tina_clear', _i32, [_vp, _vp]),
     ('tina_raster', _i32, [_vp, _vp, _i32, _u32, _u32, _vp]),
+    ('tina_raster_particles', _i32, [_vp, _vp, _vp, _i32, _u32, _u32, _vp]),
     ('tina_shade', _i32, [_vp, ctypes.POINTER(Mesh), ctypes.POINTER(Material), ctypes.POINTER(Lights),
                           ctypes.POINTER(IBLTables), _vp, ctypes.POINTER(ctypes.c_float * 3), _u32, _vp]),
     ('tina_fill', _i32, [_vp, _vp, ctypes.POINTER(ctypes.c_float * 3), _u32, _vp]),

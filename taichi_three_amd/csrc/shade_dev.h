@@ -373,6 +373,89 @@ __device__ __forceinline__ void fragment(const ShadeArgs& a, uint32_t f, int px,
   }
 }
 
+// ---- particles (tina/core/particle.py) ----------------------------------------
+
+// mapply_dir (tina/common.py:186-189): mapply with weight 0, no division.
+__device__ __forceinline__ V3 mapply_dir(const float* __restrict__ M, V3 d) {
+  float r[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float acc = M[4 * i + 3] * 0.0f;
+    acc = acc + M[4 * i + 0] * d.x;
+    acc = acc + M[4 * i + 1] * d.y;
+    acc = acc + M[4 * i + 2] * d.z;
+    r[i] = acc;
+  }
+  return v3(r[0], r[1], r[2]);
+}
+
+// What render_occup keeps of particle i (particle.py:82-104): centre Al, Rl**2,
+// view-space z, the depth int(z * 2**30) and the pixel box clamped to the screen.
+struct PSetup {
+  V3 Al;
+  float r2, vz;
+  int32_t depth, botx, boty, topx, topy;
+};
+
+// false when the particle is clipped (z outside [-1, 1] with clip) or its box is empty
+__device__ __forceinline__ bool particle_setup(const Cam& cam, gf32* pos, gf32* rad, long i, bool clip, PSetup& s) {
+  const V3 Al = ld3(pos + 3 * i);
+  const float Rl = rad[i];
+  const V3 Av = mapply_pos(cam.W2V, Al);
+  if (clip && !(-1.0f <= Av.z && Av.z <= 1.0f)) return false;
+  const V3 DXl = vnormalized(mapply_dir(cam.V2W, v3(1.0f, 0.0f, 0.0f)));
+  const V3 DYl = vnormalized(mapply_dir(cam.V2W, v3(0.0f, 1.0f, 0.0f)));
+  const float Rvx = mapply_pos(cam.W2V, vadd(Al, vsmul(Rl, DXl))).x - Av.x;
+  const float Rvy = mapply_pos(cam.W2V, vadd(Al, vsmul(Rl, DYl))).y - Av.y;
+  const float W = (float)cam.W, H = (float)cam.H;
+  // to_viewport of Av -/+ (Rv.x, 0, 0) and Av -/+ (0, Rv.y, 0)
+  const float b0x = ((Av.x - Rvx) * 0.5f + 0.5f) * W, b0y = ((Av.y - 0.0f) * 0.5f + 0.5f) * H;
+  const float b1x = ((Av.x + Rvx) * 0.5f + 0.5f) * W, b1y = ((Av.y + 0.0f) * 0.5f + 0.5f) * H;
+  const float b2x = ((Av.x - 0.0f) * 0.5f + 0.5f) * W, b2y = ((Av.y - Rvy) * 0.5f + 0.5f) * H;
+  const float b3x = ((Av.x + 0.0f) * 0.5f + 0.5f) * W, b3y = ((Av.y + Rvy) * 0.5f + 0.5f) * H;
+  s.botx = max(f2i_sat(floorf(fminf(b0x, b2x))), 0);
+  s.boty = max(f2i_sat(floorf(fminf(b0y, b2y))), 0);
+  s.topx = min(f2i_sat(ceilf(fmaxf(b1x, b3x))), cam.W - 1);
+  s.topy = min(f2i_sat(ceilf(fmaxf(b1y, b3y))), cam.H - 1);
+  s.Al = Al;
+  s.r2 = Rl * Rl;
+  s.vz = Av.z;
+  s.depth = f2i_sat(Av.z * (float)TINA_MAXDEPTH);
+  return s.topx >= s.botx && s.topy >= s.boty;
+}
+
+// Pl = from_viewspace(from_viewport(P + bias), z) (particle.py:106-108, 134-136)
+__device__ __forceinline__ V3 particle_plane_point(const Cam& cam, int px, int py, float vz) {
+  const float p_x = (float)px + cam.bias[0], p_y = (float)py + cam.bias[1];
+  const V3 Pv = v3(((p_x / (float)cam.W) * 2.0f) - 1.0f, ((p_y / (float)cam.H) * 2.0f) - 1.0f, vz);
+  return mapply_pos(cam.V2W, Pv);
+}
+
+// the disc test of particle.py:109: covered unless |Pl - Al|^2 > Rl**2
+__device__ __forceinline__ bool particle_covers(const Cam& cam, const PSetup& s, int px, int py) {
+  const V3 d = vsub(particle_plane_point(cam, px, py, s.vz), s.Al);
+  return !(vdot(d, d) > s.r2);
+}
+
+// render_color of particle f at pixel (px, py) (particle.py:125-146): the point
+// of the front hemisphere behind the pixel, its normal, the particle's colour.
+__device__ __forceinline__ void particle_fragment(const ShadeArgs& a, uint32_t f, int px, int py, Frag& fr) {
+  const V3 Al = ld3(gptr(a.mesh.verts) + 3 * (size_t)f);
+  const float Rl = gptr(a.mesh.norms)[f];
+  const V3 Av = mapply_pos(a.cam.W2V, Al);
+  const V3 Pl = particle_plane_point(a.cam, px, py, Av.z);
+  V3 Dl = vdivs(vsub(Pl, Al), Rl);
+  const V3 Zl = vnormalized(mapply_dir(a.cam.V2W, v3(0.0f, 0.0f, 1.0f)));
+  Dl = vsub(Dl, vsmul(sqrtf(1.0f - vdot(Dl, Dl)), Zl));
+  Dl = vnormalized(Dl);
+  fr.nrm = Dl;
+  fr.pos = vadd(Al, vsmul(Rl, Dl));
+  fr.tc = vs(0.0f);
+  fr.color = vs(1.0f);
+  if ((a.flags & TINA_VCOLOR) && a.mesh.colors) fr.color = ld3(gptr(a.mesh.colors) + 3 * (size_t)f);
+  fr.w0 = fr.w1 = fr.w2 = 0.0f;
+}
+
 // calc_viewdir (shader.py:81-93): -normalize(V2W(uv, +1) - V2W(uv, -1))
 __device__ __forceinline__ V3 view_ray(const Cam& cam, int px, int py) {
   const float p_x = (float)px + cam.bias[0], p_y = (float)py + cam.bias[1];
@@ -392,7 +475,8 @@ __device__ __forceinline__ V3 view_ray(const Cam& cam, int px, int py) {
 // For these the kernel is compiled with the program fixed, so no parameter
 // array and no interpreter stack are needed; the arithmetic (and its order) is
 // the one run_material performs for the same program.
-enum Prog { PROG_GENERIC = 0, PROG_DIFFUSE = 1, PROG_CLASSIC = 2, PROG_BLINN = 3, PROG_PBR = 4 };
+// PROG_PARTICLE is the generic program over particle fragments.
+enum Prog { PROG_GENERIC = 0, PROG_DIFFUSE = 1, PROG_CLASSIC = 2, PROG_BLINN = 3, PROG_PBR = 4, PROG_PARTICLE = 5 };
 
 // Value of a non-derived parameter slot (everything but FRESNEL and TEXTURE).
 __device__ __forceinline__ V3 eval_basic(const tina_param& p, uint32_t f, const Frag& fr) {
@@ -508,10 +592,11 @@ __device__ __forceinline__ V3 shade_core(const ShadeArgs& a, V3 n, V3 pos, V3 V,
 template <int PROG, bool IBL>
 __device__ __forceinline__ V3 shade_pixel(const ShadeArgs& a, uint32_t f, int px, int py) {
   Frag fr;
-  fragment(a, f, px, py, fr);
+  if constexpr (PROG == PROG_PARTICLE) particle_fragment(a, f, px, py, fr);
+  else fragment(a, f, px, py, fr);
   const V3 viewdir = vneg(view_ray(a.cam, px, py));
   V3 res;
-  if constexpr (PROG == PROG_GENERIC) {
+  if constexpr (PROG == PROG_GENERIC || PROG == PROG_PARTICLE) {
     Slots pv;
     eval_params(a.mat, f, fr.pos, fr.nrm, fr.tc, fr.color, pv);
     res = lighting(a.mat, pv, a.lights, a.ibl, IBL ? (a.flags | TINA_IBL) : (a.flags & ~TINA_IBL), fr.pos,

@@ -132,6 +132,7 @@ inline void dispatch_shade(int prog, bool ibl, unsigned grid, hipStream_t s, con
     switch (prog) {
       case PROG_DIFFUSE: launch_shade<PROG_DIFFUSE, true>(grid, s, a, keys, img); break;
       case PROG_PBR: launch_shade<PROG_PBR, true>(grid, s, a, keys, img); break;
+      case PROG_PARTICLE: launch_shade<PROG_PARTICLE, true>(grid, s, a, keys, img); break;
       default: launch_shade<PROG_GENERIC, true>(grid, s, a, keys, img); break;
     }
   } else {
@@ -140,6 +141,7 @@ inline void dispatch_shade(int prog, bool ibl, unsigned grid, hipStream_t s, con
       case PROG_CLASSIC: launch_shade<PROG_CLASSIC, false>(grid, s, a, keys, img); break;
       case PROG_BLINN: launch_shade<PROG_BLINN, false>(grid, s, a, keys, img); break;
       case PROG_PBR: launch_shade<PROG_PBR, false>(grid, s, a, keys, img); break;
+      case PROG_PARTICLE: launch_shade<PROG_PARTICLE, false>(grid, s, a, keys, img); break;
       default: launch_shade<PROG_GENERIC, false>(grid, s, a, keys, img); break;
     }
   }

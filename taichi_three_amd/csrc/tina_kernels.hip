@@ -21,6 +21,7 @@
 #include "nn_shade.h"
 #include "shade_kernels.h"
 #include "postp_kernels.h"
+#include "particle_kernels.h"
 
 using namespace tina;
 
@@ -391,6 +392,23 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
   return TINA_OK;
 }
 
+int tina_raster_particles(tina_ctx* c, const float* d_pos, const float* d_radius, int n, uint32_t id_base,
+                          uint32_t flags, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (n < 0) return fail(c, TINA_ERR_INVALID, "n must be >= 0");
+  if (n == 0) return TINA_OK;
+  if (!d_pos || !d_radius) return fail(c, TINA_ERR_INVALID, "particle positions or radii are NULL");
+  if ((uint64_t)id_base + (uint64_t)n > 0xFFFFFFFFull) return fail(c, TINA_ERR_INVALID, "particle ids overflow 32 bits");
+  if (flags & ~TINA_CLIP) return fail(c, TINA_ERR_INVALID, "particle raster flags: only TINA_CLIP applies");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = flush_clear(c, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_particles, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, s, d_pos, d_radius, n, id_base,
+                     flags, make_cam(c), (unsigned long long*)c->keys);
+  TINA_LAUNCH(c, "k_particles");
+  return TINA_OK;
+}
+
 int tina_shade(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,
                const tina_ibl* ibl, float* d_img, const float bg[3], uint32_t flags, void* stream) {
   if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
@@ -399,6 +417,10 @@ int tina_shade(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, con
   if ((flags & TINA_SMOOTH) && !mesh->norms) return fail(c, TINA_ERR_INVALID, "TINA_SMOOTH without normals");
   if ((flags & TINA_TEX) && !mesh->coors) return fail(c, TINA_ERR_INVALID, "TINA_TEX without texcoords");
   if ((flags & TINA_VCOLOR) && !mesh->colors) return fail(c, TINA_ERR_INVALID, "TINA_VCOLOR without colours");
+  if ((flags & TINA_PARTICLES) && (flags & (TINA_SMOOTH | TINA_TEX)))
+    return fail(c, TINA_ERR_INVALID, "particles take neither TINA_SMOOTH nor TINA_TEX");
+  if ((flags & TINA_PARTICLES) && mesh->nfaces > 0 && !mesh->norms)
+    return fail(c, TINA_ERR_INVALID, "TINA_PARTICLES needs the radii in mesh->norms");
   int rc = check_material(c, mat);
   if (rc) return rc;
   if (!lights && !(flags & TINA_NOLIGHT)) return fail(c, TINA_ERR_INVALID, "lights is NULL");
@@ -408,7 +430,8 @@ int tina_shade(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, con
   if ((rc = flush_clear(c, (hipStream_t)stream))) return rc;
   const long npix = (long)c->W * c->H;
   const bool ibl_mode = (flags & TINA_IBL) != 0;
-  dispatch_shade(classify_program(a.mat, ibl_mode), ibl_mode, grid_for(npix, kShadePix, 0x7fffffff),
+  const int prog = (flags & TINA_PARTICLES) ? (int)PROG_PARTICLE : classify_program(a.mat, ibl_mode);
+  dispatch_shade(prog, ibl_mode, grid_for(npix, kShadePix, 0x7fffffff),
                  (hipStream_t)stream, a, c->keys, d_img);
   TINA_LAUNCH(c, "k_shade");
   return TINA_OK;
@@ -595,8 +618,9 @@ int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, 
   if (mesh->nfaces < 0) return fail(c, TINA_ERR_INVALID, "nfaces must be >= 0");
   if ((flags & TINA_SMOOTH) && !mesh->norms) return fail(c, TINA_ERR_INVALID, "TINA_SMOOTH without normals");
   if ((flags & TINA_VCOLOR) && !mesh->colors) return fail(c, TINA_ERR_INVALID, "TINA_VCOLOR without colours");
-  if ((flags & (TINA_IBL | TINA_TONEMAP)) != (TINA_IBL | TINA_TONEMAP) || (flags & (TINA_NOLIGHT | TINA_TEX)))
-    return fail(c, TINA_ERR_INVALID, "the surrogate shades IBL + lights with ACES, without textures");
+  if ((flags & (TINA_IBL | TINA_TONEMAP)) != (TINA_IBL | TINA_TONEMAP) ||
+      (flags & (TINA_NOLIGHT | TINA_TEX | TINA_PARTICLES)))
+    return fail(c, TINA_ERR_INVALID, "the surrogate shades triangle meshes, IBL + lights with ACES, without textures");
   int rc = check_material(c, mat);
   if (rc) return rc;
   if ((rc = check_sun(c, lights))) return rc;
@@ -747,7 +771,7 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
   for (int i = 0; i < nscenes; i++) {
     const tina_scene_desc& d = scenes[i];
     if (d.width <= 0 || d.height <= 0 || d.width >= 32768 || d.height >= 32768 || d.nfaces < 0 || !d.mesh.verts ||
-        !d.img || d.mesh.nfaces != d.nfaces || d.mesh.id_base != 0) {
+        !d.img || d.mesh.nfaces != d.nfaces || d.mesh.id_base != 0 || (d.shade_flags & TINA_PARTICLES)) {
       batch_free(b);
       return fail(nullptr, TINA_ERR_INVALID, "scene " + std::to_string(i) + ": malformed descriptor");
     }

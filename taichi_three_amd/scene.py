@@ -16,6 +16,7 @@ from .lighting import Lighting, SkyboxLighting
 from .matr import Diffuse
 from .postp import ToneMapping, Blooming, FXAA, Accumator
 from .raster import TriangleRaster
+from .particle import ParticleRaster
 from .shader import Shader, ShaderGroup
 
 __all__ = ['Scene']
@@ -85,7 +86,7 @@ class Scene:
         self.shaders[material] = ShaderGroup(self.pre_shaders + [shader] + self.post_shaders)
 
     def add_object(self, object, material=None, raster=None):
-        """Adds a mesh; the raster is picked by duck typing (raster.py:112-146)."""
+        """Adds a mesh or a particle set; the raster is picked by duck typing (raster.py:112-146)."""
         assert object not in self.objects
         if material is None:
             material = self.default_material
@@ -94,8 +95,12 @@ class Scene:
                 if not hasattr(self, 'triangle_raster'):
                     self.triangle_raster = TriangleRaster(self.engine, **self.options)
                 raster = self.triangle_raster
-            elif hasattr(object, 'get_nfaces') or hasattr(object, 'get_npars') or hasattr(object, 'sample_volume'):
-                raise NotImplementedError(f'{type(object).__name__}: only triangle meshes are rasterised here')
+            elif hasattr(object, 'get_npars'):
+                if not hasattr(self, 'particle_raster'):
+                    self.particle_raster = ParticleRaster(self.engine, **self.options)
+                raster = self.particle_raster
+            elif hasattr(object, 'get_nfaces') or hasattr(object, 'sample_volume'):
+                raise NotImplementedError(f'{type(object).__name__}: wireframe and volume rasterisers are not provided')
             else:
                 raise ValueError(f'cannot determine raster type of object: {object}')
         self._ensure_material_shader(material)
