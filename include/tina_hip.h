@@ -190,6 +190,21 @@ int tina_raster(tina_ctx *ctx, const float *d_verts, int nfaces, uint32_t id_bas
 int tina_raster_particles(tina_ctx *ctx, const float *d_pos, const float *d_radius, int n,
                           uint32_t id_base, uint32_t flags, void *stream);
 
+/* WireframeRaster.render_color's coverage (tina/core/wireframe.py:49-95): the
+ * samples draw_line steps along the screen segment of each wire i in [0, n)
+ * (d_verts [n][2][3], the two end points) enter the depth|id min with the
+ * perspective-interpolated depth and id id_base + i.  flags: TINA_CLIP drops a
+ * wire whose two end points both leave the view cube (wireframe.py:75-78). */
+int tina_raster_wires(tina_ctx *ctx, const float *d_verts, int n, uint32_t id_base, uint32_t flags,
+                      void *stream);
+
+/* The colour pass of WireframeRaster (wireframe.py:92-95, Shader.blend_color
+ * with factor 1): every pixel whose winning id lies in [id_base, id_base + n)
+ * takes color.  With TINA_BG the pixels no id won take bg; TINA_TONEMAP maps
+ * what is written through ACES.  Other flags are rejected. */
+int tina_paint_ids(tina_ctx *ctx, uint32_t id_base, int n, const float color[3], const float bg[3],
+                   uint32_t flags, float *d_img, void *stream);
+
 /* TriangleRaster.render_color + Shader/Lighting/material (triangle.py:133-153,
  * shader.py:112-131, lighting.py:84-98): shades the pixels whose winning id lies
  * in [mesh->id_base, mesh->id_base + mesh->nfaces) into d_img ([H][W][3] f32).

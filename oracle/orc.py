@@ -69,6 +69,18 @@ def raster_particles(W, H, cam, pos, radius, id_base=0, flags=2, depth=None, occ
     return depth, occup
 
 
+def raster_wires(W, H, cam, verts, id_base=0, flags=0, depth=None, occup=None):
+    """Serial wire depth pass (orc_raster_wires); verts [n, 2, 3].  Returns (depth, occup)."""
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 2, 3)
+    if depth is None:
+        depth = np.empty((H, W), dtype=np.int32)
+        occup = np.empty((H, W), dtype=np.int32)
+        lib().orc_clear(W, H, _p(depth), _p(occup))
+    lib().orc_raster_wires(W, H, ctypes.byref(cam), _p(verts), len(verts), ctypes.c_uint32(id_base),
+                           ctypes.c_uint32(flags), _p(depth), _p(occup))
+    return depth, occup
+
+
 def particle_mesh(pos, radius, colors=None, id_base=0):
     """HostMesh of a particle set for shade(..., flags | PARTICLES): centres in
     verts, radii in norms, colours in colors."""

@@ -245,6 +245,66 @@ void orc_raster_particles(int W, int H, const tina_camera *cam, const float *pos
     }
 }
 
+/* ---- wires: tina/core/wireframe.py:49-95 (draw_line + render_color's depth test) ---- */
+
+static inline int in_cube(const float v[3])
+{
+    int ok = 1;
+    for (int i = 0; i < 3; i++) ok &= (-1.0f <= v[i]) & (v[i] <= 1.0f);
+    return ok;
+}
+
+void orc_raster_wires(int W, int H, const tina_camera *cam, const float *verts, int n, uint32_t id_base,
+                      uint32_t flags, int32_t *depth, int32_t *occup)
+{
+    const float res[2] = {(float)W, (float)H};
+    for (int f = 0; f < n; f++) {
+        const float *Al = verts + 6L * f, *Bl = verts + 6L * f + 3;
+        float r[2][3], w[2], Av[3], Bv[3];
+        mapply1(cam->W2V, Al, r[0], &w[0]);
+        mapply1(cam->W2V, Bl, r[1], &w[1]);
+        for (int i = 0; i < 3; i++) { Av[i] = r[0][i] / w[0]; Bv[i] = r[1][i] / w[1]; }
+        if ((flags & TINA_CLIP) && !in_cube(Av) && !in_cube(Bv)) continue;    /* :75-78 */
+        float a[2], b[2];
+        for (int i = 0; i < 2; i++) {                                       /* :80 */
+            a[i] = (Av[i] * 0.5f + 0.5f) * res[i];
+            b[i] = (Bv[i] * 0.5f + 0.5f) * res[i];
+        }
+        const float ws[2] = {1.0f / w[0], 1.0f / w[1]};                     /* :83 */
+        /* draw_line(a, b) (:50-64) */
+        const float dlt[2] = {b[0] - a[0], b[1] - a[1]};
+        const float adlt[2] = {fabsf(dlt[0]), fabsf(dlt[1])};
+        float k[2];
+        int32_t siz;
+        if (adlt[0] >= adlt[1]) {
+            k[0] = dlt[0] >= 0.0f ? 1.0f : -1.0f;
+            k[1] = (k[0] * dlt[1]) / dlt[0];
+            siz = f2i_sat(adlt[0]);
+        } else {
+            k[1] = dlt[1] >= 0.0f ? 1.0f : -1.0f;
+            k[0] = (k[1] * dlt[0]) / dlt[1];
+            siz = f2i_sat(adlt[1]);
+        }
+        for (long i = 0; i <= (long)siz; i++) {
+            const float fi = (float)i;
+            const float pos[2] = {(a[0] + k[0] * fi) + cam->bias[0], (a[1] + k[1] * fi) + cam->bias[1]};
+            const float cor = fi / (float)siz;
+            const int32_t X = f2i_sat(floorf(pos[0])), Y = f2i_sat(floorf(pos[1]));
+            if (!(0 <= X && X < W && 0 <= Y && Y < H)) continue;            /* :87 */
+            float w0 = (1.0f - cor) * ws[0], w1 = cor * ws[1];              /* :88-89 */
+            const float sum = w0 + w1;
+            w0 = w0 / sum;
+            w1 = w1 / sum;
+            const int32_t d = f2i_sat(((w0 * Av[2]) + (w1 * Bv[2])) * 1073741824.0f);   /* :90-91 */
+            const long P = (long)Y * W + X;
+            if (d < depth[P]) {          /* serial form of atomic_min (:92-93) */
+                depth[P] = d;
+                occup[P] = (int32_t)(id_base + (uint32_t)f);
+            }
+        }
+    }
+}
+
 static inline uint64_t pack_key(int32_t depth, uint32_t id)
 {
     return ((uint64_t)((uint32_t)depth ^ 0x80000000u) << 32) | (uint64_t)id;

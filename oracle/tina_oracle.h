@@ -18,6 +18,7 @@
  *   orc_ibl_*         tina/core/lighting.py:4-21, material.py:251-320,407-442,
  *                     tina/skybox.py:23-145, tina/common.py:140-149 (bilerp)
  *   orc_raster_particles  tina/core/particle.py:77-116 (render_occup), particles ascending
+ *   orc_raster_wires  tina/core/wireframe.py:49-95 (draw_line steps), wires ascending
  *   orc_raster_omp    the same coverage rule, parallel over faces with a 64-bit
  *                     compare-and-swap min (the CPU baseline; same result)
  *
@@ -41,6 +42,11 @@ void orc_raster(int W, int H, const tina_camera *cam, const float *verts, int T,
  * particles: pos [n][3], radius [n]; flags: TINA_CLIP. */
 void orc_raster_particles(int W, int H, const tina_camera *cam, const float *pos, const float *radius, int n,
                           uint32_t id_base, uint32_t flags, int32_t *depth, int32_t *occup);
+
+/* WireframeRaster's depth pass (tina/core/wireframe.py:49-95), serial over wires
+ * and over the steps of draw_line: verts [n][2][3]; flags: TINA_CLIP. */
+void orc_raster_wires(int W, int H, const tina_camera *cam, const float *verts, int n, uint32_t id_base,
+                      uint32_t flags, int32_t *depth, int32_t *occup);
 
 /* Parallel coverage into packed keys ([H][W] u64), nthreads OpenMP threads. */
 void orc_clear_keys(int W, int H, uint64_t *keys, int nthreads);

@@ -18,6 +18,7 @@ from .shader import *
 from .raster import TriangleRaster
 from .pars import *
 from .particle import ParticleRaster
+from .wireframe import WireframeRaster
 from .postp import ToneMapping, Blooming, FXAA, Accumator
 from .probe import ProbeShader
 from .scene import Scene

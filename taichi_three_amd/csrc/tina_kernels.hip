@@ -22,6 +22,7 @@
 #include "shade_kernels.h"
 #include "postp_kernels.h"
 #include "particle_kernels.h"
+#include "wire_kernels.h"
 
 using namespace tina;
 
@@ -403,9 +404,45 @@ int tina_raster_particles(tina_ctx* c, const float* d_pos, const float* d_radius
   hipStream_t s = (hipStream_t)stream;
   int rc = flush_clear(c, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_particles, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, s, d_pos, d_radius, n, id_base,
-                     flags, make_cam(c), (unsigned long long*)c->keys);
+  constexpr int per_block = kBlock / kParWalkWaves;  // particles per block pass
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_particles<kParWalkWaves, kParPrecheck>), dim3(grid_for(n, per_block, 16384)),
+                     dim3(kBlock), 0, s, d_pos, d_radius, n, id_base, flags, make_cam(c), (unsigned long long*)c->keys);
   TINA_LAUNCH(c, "k_particles");
+  return TINA_OK;
+}
+
+int tina_raster_wires(tina_ctx* c, const float* d_verts, int n, uint32_t id_base, uint32_t flags, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (n < 0) return fail(c, TINA_ERR_INVALID, "n must be >= 0");
+  if (n == 0) return TINA_OK;
+  if (!d_verts) return fail(c, TINA_ERR_INVALID, "wire vertices are NULL");
+  if ((uint64_t)id_base + (uint64_t)n > 0xFFFFFFFFull) return fail(c, TINA_ERR_INVALID, "wire ids overflow 32 bits");
+  if (flags & ~TINA_CLIP) return fail(c, TINA_ERR_INVALID, "wire raster flags: only TINA_CLIP applies");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = flush_clear(c, s);
+  if (rc) return rc;
+  constexpr int per_block = kBlock / kWireWalkWaves;
+  hipLaunchKernelGGL(k_wires, dim3(grid_for(n, per_block, 16384)), dim3(kBlock), 0, s, d_verts, n, id_base, flags,
+                     make_cam(c), (unsigned long long*)c->keys);
+  TINA_LAUNCH(c, "k_wires");
+  return TINA_OK;
+}
+
+int tina_paint_ids(tina_ctx* c, uint32_t id_base, int n, const float color[3], const float bg[3], uint32_t flags,
+                   float* d_img, void* stream) {
+  if (!c || !d_img || !color) return fail(c, TINA_ERR_INVALID, "ctx, image or colour is NULL");
+  if (n < 0) return fail(c, TINA_ERR_INVALID, "n must be >= 0");
+  if (flags & ~(TINA_BG | TINA_TONEMAP)) return fail(c, TINA_ERR_INVALID, "paint flags: TINA_BG and TINA_TONEMAP only");
+  if ((flags & TINA_BG) && !bg) return fail(c, TINA_ERR_INVALID, "TINA_BG without a background colour");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = flush_clear(c, s);
+  if (rc) return rc;
+  const long npix = (long)c->W * c->H;
+  const float* b = (flags & TINA_BG) ? bg : color;
+  hipLaunchKernelGGL(k_paint, dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, (const uint64_t*)c->keys, npix,
+                     id_base, (uint32_t)n, color[0], color[1], color[2], b[0], b[1], b[2], (flags & TINA_BG) ? 1 : 0,
+                     (flags & TINA_TONEMAP) ? 1 : 0, d_img);
+  TINA_LAUNCH(c, "k_paint");
   return TINA_OK;
 }
 

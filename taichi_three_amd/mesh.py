@@ -15,7 +15,7 @@ import numpy as np
 from .engine import MAX
 
 __all__ = ['SimpleMesh', 'PrimitiveMesh', 'MeshFlatNormal', 'MeshSmoothNormal', 'ConnectiveMesh',
-           'MeshTransform', 'MeshEditBase', 'MeshModel']
+           'MeshTransform', 'MeshEditBase', 'MeshModel', 'MeshToWire']
 
 
 def _torch():
@@ -39,8 +39,8 @@ class SimpleMesh:
     """Face-soup mesh (tina/mesh/simple.py:4-115)."""
 
     def __init__(self, maxfaces=MAX, npolygon=3):
-        if npolygon != 3:
-            raise NotImplementedError('only triangle meshes (npolygon=3) are rasterised here')
+        if npolygon not in (2, 3):
+            raise NotImplementedError('meshes of triangles (npolygon=3) or wires (npolygon=2) are rasterised here')
         self.maxfaces = int(maxfaces)
         self.npolygon = npolygon
         self.nfaces = 0
@@ -67,16 +67,16 @@ class SimpleMesh:
     def set_face_verts(self, verts):
         """verts: [nfaces, 3, 3]; the face count is verts.shape[0] (simple.py:53-67)."""
         n = self._n(verts)
-        self.verts = _as_device(verts[:n], (3, 3))
+        self.verts = _as_device(verts[:n], (self.npolygon, 3))
         self.nfaces = n
         self.version += 1
 
     def set_face_norms(self, norms):
-        self.norms = _as_device(norms[:self.nfaces], (3, 3))
+        self.norms = _as_device(norms[:self.nfaces], (self.npolygon, 3))
         self.version += 1
 
     def set_face_coors(self, coors):
-        self.coors = _as_device(coors[:self.nfaces], (3, 2))
+        self.coors = _as_device(coors[:self.nfaces], (self.npolygon, 2))
         self.version += 1
 
     def set_face_mtlids(self, mtlids):
@@ -87,7 +87,7 @@ class SimpleMesh:
 
     def set_face_colors(self, colors):
         """Per-vertex colours [nfaces, 3, 3] bound to Input('color') (extension)."""
-        self.colors = _as_device(colors[:self.nfaces], (3, 3))
+        self.colors = _as_device(colors[:self.nfaces], (self.npolygon, 3))
         self.version += 1
 
     def set_face_attr(self, name, values):
@@ -421,3 +421,36 @@ class MeshTransform(MeshEditBase):
             N = torch.as_tensor(np.linalg.inv(self.trans)[:3, :3].T.astype(np.float32), device=v.device)
             fb['norms'] = (fb['norms'] @ N.T).contiguous()
         return fb
+
+
+class MeshToWire(MeshEditBase):
+    """The edges of every polygon as wires (tina/mesh/wire.py:5-39): wire
+    n = f * npolygon + e joins corner e of face f to corner (e + 1) % npolygon."""
+
+    def get_npolygon(self):
+        return 2
+
+    def _src_npolygon(self):
+        return self.mesh.get_npolygon() if hasattr(self.mesh, 'get_npolygon') else 3
+
+    def get_nfaces(self):
+        return self.mesh.get_nfaces() * self._src_npolygon()
+
+    def face_buffers(self):
+        fb = dict(self.mesh.face_buffers())
+        k = self._src_npolygon()
+        torch = _torch()
+        nxt = torch.tensor([(e + 1) % k for e in range(k)], device=fb['verts'].device) if fb['verts'] is not None \
+            else None
+
+        def edges(a):
+            if a is None:
+                return None
+            return torch.stack([a, a[:, nxt]], dim=2).reshape(-1, 2, a.shape[-1]).contiguous()
+
+        out = dict(fb)
+        for key in ('verts', 'norms', 'coors', 'colors'):
+            out[key] = edges(fb.get(key))
+        out['nfaces'] = int(fb['nfaces']) * k
+        out['attrs'] = {}
+        return out

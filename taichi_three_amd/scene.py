@@ -17,6 +17,7 @@ from .matr import Diffuse
 from .postp import ToneMapping, Blooming, FXAA, Accumator
 from .raster import TriangleRaster
 from .particle import ParticleRaster
+from .wireframe import WireframeRaster
 from .shader import Shader, ShaderGroup
 
 __all__ = ['Scene']
@@ -86,21 +87,26 @@ class Scene:
         self.shaders[material] = ShaderGroup(self.pre_shaders + [shader] + self.post_shaders)
 
     def add_object(self, object, material=None, raster=None):
-        """Adds a mesh or a particle set; the raster is picked by duck typing (raster.py:112-146)."""
+        """Adds a mesh, a wire mesh or a particle set; the raster is picked by duck typing (raster.py:112-146)."""
         assert object not in self.objects
         if material is None:
             material = self.default_material
         if raster is None:
-            if hasattr(object, 'get_nfaces') and not (hasattr(object, 'get_npolygon') and object.get_npolygon() == 2):
-                if not hasattr(self, 'triangle_raster'):
-                    self.triangle_raster = TriangleRaster(self.engine, **self.options)
-                raster = self.triangle_raster
+            if hasattr(object, 'get_nfaces'):
+                if hasattr(object, 'get_npolygon') and object.get_npolygon() == 2:
+                    if not hasattr(self, 'wireframe_raster'):
+                        self.wireframe_raster = WireframeRaster(self.engine, **self.options)
+                    raster = self.wireframe_raster
+                else:
+                    if not hasattr(self, 'triangle_raster'):
+                        self.triangle_raster = TriangleRaster(self.engine, **self.options)
+                    raster = self.triangle_raster
             elif hasattr(object, 'get_npars'):
                 if not hasattr(self, 'particle_raster'):
                     self.particle_raster = ParticleRaster(self.engine, **self.options)
                 raster = self.particle_raster
-            elif hasattr(object, 'get_nfaces') or hasattr(object, 'sample_volume'):
-                raise NotImplementedError(f'{type(object).__name__}: wireframe and volume rasterisers are not provided')
+            elif hasattr(object, 'sample_volume'):
+                raise NotImplementedError(f'{type(object).__name__}: the volume raster is not provided')
             else:
                 raise ValueError(f'cannot determine raster type of object: {object}')
         self._ensure_material_shader(material)
@@ -157,6 +163,9 @@ class Scene:
             self._fill_background(tm)
             return
         first = True
+        if isinstance(order[0][1].raster, WireframeRaster) and isinstance(self.lighting, SkyboxLighting):
+            self._fill_background(tm)   # the paint pass has no skybox: the sky goes first
+            first = False
         for obj, info in order:
             r = info.raster
             r._fb, r._mesh, r.nfaces, r.id_base = info.fb, obj, info.nfaces, info.id_base
