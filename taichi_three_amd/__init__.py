@@ -12,6 +12,7 @@ from .field import Field
 from .util.matrix import *
 from .mesh import *
 from .assimp import *
+from .gltf import readgltf
 from .matr import *
 from .lighting import *
 from .shader import *

@@ -111,9 +111,14 @@ class Input(Node):
 
 
 def _load_image(path):
-    """ti.imread + texture_as_field (tina/advans.py:8-28): [x, y, c] with y up, u8 -> /255."""
-    from PIL import Image
-    img = np.asarray(Image.open(path))
+    """ti.imread + texture_as_field (tina/advans.py:8-28): [x, y, c] with y up, u8 -> /255.
+    PNG files are read without PIL when it is not installed."""
+    try:
+        from PIL import Image
+        img = np.asarray(Image.open(path))
+    except ImportError:
+        from .pngread import read_png
+        img = read_png(path)
     img = np.ascontiguousarray(np.transpose(img, (1, 0, 2) if img.ndim == 3 else (1, 0))[:, ::-1])
     if img.dtype == np.uint8:
         img = np.float32(img / 255)
@@ -128,7 +133,11 @@ class Texture(Node):
 
     def __init__(self, path, **kwargs):
         import torch
-        arr = _load_image(path) if isinstance(path, str) else np.asarray(path, dtype=np.float32)
+        if isinstance(path, str):
+            arr = _load_image(path)
+        else:
+            arr = np.asarray(path)
+            arr = np.float32(arr / 255) if arr.dtype == np.uint8 else arr.astype(np.float32)  # advans.py:13-14
         if arr.ndim == 2:
             arr = arr[:, :, None]
         self.shape = arr.shape

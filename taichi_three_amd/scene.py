@@ -188,6 +188,11 @@ class Scene:
         _abi.check(self.engine._lib.tina_fill(self.engine._ctx, _abi.ptr(self.image.storage), ctypes.byref(bg),
                                               tm, _abi.stream_handle()), self.engine._ctx)
 
+    def load_gltf(self, path):
+        """Adds the objects of a glTF file to the scene (raster.py:234-241); returns the scene."""
+        from .gltf import readgltf
+        return readgltf(path).extract(self)
+
     @property
     def img(self):
         """The final (tone-mapped) image, a Field of shape (W, H, 3); with taa the
