@@ -201,6 +201,59 @@ def post_passes(reps):
     return out
 
 
+def other_rasters(reps):
+    """Particle and wire passes beside the triangle frame (device ms, median):
+    1 M particles of radius 0.003-0.008 at 1920x1080 and 64 k of radius
+    0.01-0.02 at 512x512 (the particle example's size), uniform in [-1, 1]^3,
+    Diffuse, camera at distance 3.2 with a 45 degree field of view; 200 k
+    random wires in [-1.2, 1.2]^3 at 1920x1080.  'raster' is the coverage
+    pass (with the key clear), 'shade' the colour pass."""
+    import numpy as np
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd.util.matrix import lookat, perspective
+    out = {}
+
+    def timed(scene, tag):
+        ev = []
+
+        def mark(name):
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            ev.append((name, e))
+        for _ in range(3):
+            scene.render()
+        torch.cuda.synchronize()
+        per = {}
+        for _ in range(reps):
+            ev.clear()
+            scene.render(on_phase=mark)
+            torch.cuda.synchronize()
+            for (a, ea), (_, eb) in zip(ev, ev[1:]):
+                per.setdefault(a, []).append(ea.elapsed_time(eb))
+        out[tag] = {k: round(float(np.median(v)), 4) for k, v in per.items() if k in ('raster', 'shade')}
+
+    rng = np.random.default_rng(0)
+    for tag, (W, H, n, rlo, rhi) in (('particles_1080p_1M', (1920, 1080, 1 << 20, 0.003, 0.008)),
+                                     ('particles_512_64k', (512, 512, 65536, 0.01, 0.02))):
+        scene = tina.Scene(W, H, maxpars=n)
+        pars = tina.SimpleParticles(n)
+        pars.set_particles(rng.uniform(-1, 1, (n, 3)).astype(np.float32))
+        pars.set_particle_radii(rng.uniform(rlo, rhi, n).astype(np.float32))
+        scene.add_object(pars, tina.Diffuse())
+        scene.set_camera(lookat([0, 0, 0], [0.4, 0.3, 3.2]), perspective(45, W / H, 0.1, 50))
+        timed(scene, tag)
+        out[tag]['particles_per_s'] = round(n / (out[tag]['raster'] + out[tag]['shade']) * 1e3, 0)
+    W, H, n = 1920, 1080, 200_000
+    scene = tina.Scene(W, H)
+    mesh = tina.SimpleMesh(n, npolygon=2)
+    mesh.set_face_verts(rng.uniform(-1.2, 1.2, (n, 2, 3)).astype(np.float32))
+    scene.add_object(mesh)
+    scene.set_camera(lookat([0, 0, 0], [0.4, 0.3, 3.2]), perspective(45, W / H, 0.1, 50))
+    timed(scene, 'wires_1080p_200k')
+    return out
+
+
 def batched_views(reps, tune_reps=3):
     """64 independent scenes (1k-500k faces log-uniform, resolutions from
     {256^2, 512^2, 1024^2, 1920x1080}, cameras on the radius-3 sphere, seed 2)
@@ -343,6 +396,7 @@ def main():
     if not args.no_batch:
         out['batch'] = batched_views(max(5, args.steps // 4))
     out['postp'] = post_passes(max(10, args.steps // 2))
+    out['other_rasters'] = other_rasters(max(10, args.steps // 2))
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(w)
         out['speedup_vs_cpu'] = round(fps / out['cpu_baseline']['value'], 1)

@@ -205,6 +205,23 @@ int tina_raster_wires(tina_ctx *ctx, const float *d_verts, int n, uint32_t id_ba
 int tina_paint_ids(tina_ctx *ctx, uint32_t id_base, int n, const float color[3], const float bg[3],
                    uint32_t flags, float *d_img, void *stream);
 
+/* NormalShader (tina/core/shader.py:43-46): writes the shading normal of every
+ * pixel whose winning id lies in [mesh->id_base, +mesh->nfaces) into d_norm
+ * ([H][W][3] f32), as the colour pass computes it.  flags: TINA_SMOOTH
+ * (interpolated vertex normals) or TINA_PARTICLES (the sphere normal). */
+int tina_normals(tina_ctx *ctx, const tina_mesh *mesh, uint32_t flags, float *d_norm, void *stream);
+
+/* SSAO.render (tina/postp/ssao.py:61-96): ambient occlusion per pixel from the
+ * depth buffer and d_norm, with nsamples hemisphere samples d_samples [n][3]
+ * turned by the nrot x nrot tile d_rot [nrot][nrot][2] (cos, sin); writes
+ * clamp(factor * (occ / n - thresh), 0, 1) to d_ao ([H][W] f32). */
+int tina_ssao(tina_ctx *ctx, const float *d_norm, const float *d_samples, int nsamples, const float *d_rot,
+              int nrot, float radius, float thresh, float factor, float *d_ao, void *stream);
+
+/* SSAO.apply (ssao.py:37-48): d_img ([H][W][3]) times 1 - (d_ao averaged over
+ * an nrot x nrot box, box = 1) or times 1 - d_ao (box = 0, the TAA mode). */
+int tina_ssao_apply(float *d_img, int width, int height, const float *d_ao, int nrot, int box, void *stream);
+
 /* TriangleRaster.render_color + Shader/Lighting/material (triangle.py:133-153,
  * shader.py:112-131, lighting.py:84-98): shades the pixels whose winning id lies
  * in [mesh->id_base, mesh->id_base + mesh->nfaces) into d_img ([H][W][3] f32).

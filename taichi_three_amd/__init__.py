@@ -20,7 +20,7 @@ from .raster import TriangleRaster
 from .pars import *
 from .particle import ParticleRaster
 from .wireframe import WireframeRaster
-from .postp import ToneMapping, Blooming, FXAA, Accumator
+from .postp import ToneMapping, Blooming, FXAA, Accumator, SSAO
 from .probe import ProbeShader
 from .scene import Scene
 from . import _abi

@@ -180,4 +180,103 @@ k_accumulate(float* __restrict__ acc, const float* __restrict__ src, long n, flo
   }
 }
 
+// ---- SSAO (tina/postp/ssao.py) and the normal buffer it reads ---------------
+
+// NormalShader.shade_color (tina/core/shader.py:43-46): the shading normal of
+// every pixel won by an id of the mesh (particles: their sphere normal).
+template <bool PART>
+__global__ void __launch_bounds__(kBlock)
+k_normals(const ShadeArgs a, const uint64_t* __restrict__ keys, float* __restrict__ norm) {
+  const long npix = (long)a.cam.W * a.cam.H;
+  for (long P = (long)blockIdx.x * kBlock + threadIdx.x; P < npix; P += (long)gridDim.x * kBlock) {
+    const uint64_t k = keys[P];
+    if (key_depth(k) >= TINA_MAXDEPTH) continue;
+    const uint32_t f = (uint32_t)k - a.mesh.id_base;
+    if (f >= (uint32_t)a.mesh.nfaces) continue;
+    Frag fr;
+    const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
+    if (PART) particle_fragment(a, f, px, py, fr);
+    else fragment(a, f, px, py, fr);
+    float* o = norm + 3 * P;
+    o[0] = fr.nrm.x;
+    o[1] = fr.nrm.y;
+    o[2] = fr.nrm.z;
+  }
+}
+
+// SSAO.render_at (ssao.py:67-96) with the fixed sample set and rotation tile of
+// the non-TAA mode (:28-35, 75-82): ao = clamp(factor * (occ / n - thresh)).
+__global__ void __launch_bounds__(kBlock)
+k_ssao(const Cam cam, const uint64_t* __restrict__ keys, const float* __restrict__ norm,
+       const float* __restrict__ samples, int nsamples, const float* __restrict__ rot, int nrot, float radius,
+       float thresh, float factor, float* __restrict__ ao_out) {
+  const long npix = (long)cam.W * cam.H;
+  const float resx = (float)cam.W, resy = (float)cam.H;
+  for (long P = (long)blockIdx.x * kBlock + threadIdx.x; P < npix; P += (long)gridDim.x * kBlock) {
+    const int x = (int)(P % cam.W), y = (int)(P / cam.W);
+    const V3 normal = ld_px(norm, cam.W, x, y);
+    const float p_x = (float)x + cam.bias[0], p_y = (float)y + cam.bias[1];
+    const V3 vpos = v3(((p_x / resx) * 2.0f) - 1.0f, ((p_y / resy) * 2.0f) - 1.0f,
+                       (float)key_depth(keys[P]) / (float)TINA_MAXDEPTH);
+    const V3 pos = mapply_pos(cam.V2W, vpos);
+    const V3 viewdir = vneg(view_ray(cam, x, y));
+    const float vradius = mapply_pos(cam.W2V, vsub(pos, vsmul(radius, viewdir))).z - vpos.z;
+    // tangentspace(normal) (tina/advans.py:97-100): columns tan, bitan, normal
+    const V3 bitan = vnormalized(vcross(normal, v3(233.0f, 666.0f, 512.0f)));
+    const V3 tan = vcross(bitan, normal);
+    const float* r = rot + 2 * ((long)(x % nrot) * nrot + (y % nrot));
+    const float rx = r[0], ry = r[1];
+    float occ = 0.0f;
+    for (int i = 0; i < nsamples; i++) {
+      const float* sp = samples + 3 * i;
+      const float sx = (rx * sp[0]) + (ry * sp[1]), sy = (-rx * sp[0]) + (ry * sp[1]), sz = sp[2];
+      V3 smp = v3(((tan.x * sx) + (bitan.x * sy)) + (normal.x * sz), ((tan.y * sx) + (bitan.y * sy)) + (normal.y * sz),
+                  ((tan.z * sx) + (bitan.z * sy)) + (normal.z * sz));
+      smp = vadd(pos, v3(smp.x * radius, smp.y * radius, smp.z * radius));
+      smp = mapply_pos(cam.W2V, smp);
+      const float Dx = (smp.x * 0.5f + 0.5f) * resx, Dy = (smp.y * 0.5f + 0.5f) * resy;
+      if (!(0.0f <= Dx && Dx < resx && 0.0f <= Dy && Dy < resy)) continue;
+      const long Q = (long)(int)Dy * cam.W + (int)Dx;
+      const float depth = (float)key_depth(keys[Q]) / (float)TINA_MAXDEPTH;
+      if (depth < smp.z) {
+        float rc = vradius / (vpos.z - depth);
+        const float t = fminf(1.0f, fmaxf(0.0f, fabsf(rc)));  // smoothstep(|rc|, 0, 1)
+        rc = (t * t) * (3.0f - 2.0f * t);
+        occ = occ + rc;
+      }
+    }
+    float ao = occ / (float)nsamples;
+    ao = factor * (ao - thresh);
+    ao_out[P] = fminf(1.0f, fmaxf(0.0f, ao));
+  }
+}
+
+// SSAO.apply (ssao.py:37-48): the image times 1 - (the ao averaged over the
+// nrot x nrot box at offset -nrot / 2, reads clamped to the screen), or times
+// 1 - ao without the box (box = 0, the TAA mode).
+__global__ void __launch_bounds__(kBlock)
+k_ssao_apply(float* __restrict__ img, const float* __restrict__ ao, int W, int H, int nrot, int box) {
+  const long npix = (long)W * H;
+  for (long P = (long)blockIdx.x * kBlock + threadIdx.x; P < npix; P += (long)gridDim.x * kBlock) {
+    const int x = (int)(P % W), y = (int)(P / W);
+    float m;
+    if (box) {
+      const int offs = nrot / 2;
+      float r = 0.0f;
+      for (int k = 0; k < nrot; k++)
+        for (int l = 0; l < nrot; l++) {
+          const int xx = min(max(x + k - offs, 0), W - 1), yy = min(max(y + l - offs, 0), H - 1);
+          r = r + ao[(long)yy * W + xx];
+        }
+      m = 1.0f - r / (float)(nrot * nrot);
+    } else {
+      m = 1.0f - ao[P];
+    }
+    float* p = img + 3 * P;
+    p[0] = p[0] * m;
+    p[1] = p[1] * m;
+    p[2] = p[2] * m;
+  }
+}
+
 }  // namespace tina

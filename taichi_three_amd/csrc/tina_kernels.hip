@@ -446,6 +446,58 @@ int tina_paint_ids(tina_ctx* c, uint32_t id_base, int n, const float color[3], c
   return TINA_OK;
 }
 
+int tina_normals(tina_ctx* c, const tina_mesh* mesh, uint32_t flags, float* d_norm, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (!mesh || !d_norm) return fail(c, TINA_ERR_INVALID, "mesh or normal buffer is NULL");
+  if (flags & ~(TINA_SMOOTH | TINA_PARTICLES)) return fail(c, TINA_ERR_INVALID, "normal flags: TINA_SMOOTH, TINA_PARTICLES");
+  if (mesh->nfaces < 0) return fail(c, TINA_ERR_INVALID, "nfaces must be >= 0");
+  if (mesh->nfaces == 0) return TINA_OK;
+  if (!mesh->verts) return fail(c, TINA_ERR_INVALID, "verts is NULL");
+  if ((flags & (TINA_SMOOTH | TINA_PARTICLES)) && !mesh->norms)
+    return fail(c, TINA_ERR_INVALID, "TINA_SMOOTH / TINA_PARTICLES need mesh->norms (normals / radii)");
+  ShadeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.mesh = *mesh;
+  a.cam = make_cam(c);
+  a.flags = flags;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = flush_clear(c, s);
+  if (rc) return rc;
+  const long npix = (long)c->W * c->H;
+  if (flags & TINA_PARTICLES)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_normals<true>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
+                       (const uint64_t*)c->keys, d_norm);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_normals<false>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
+                       (const uint64_t*)c->keys, d_norm);
+  TINA_LAUNCH(c, "k_normals");
+  return TINA_OK;
+}
+
+int tina_ssao(tina_ctx* c, const float* d_norm, const float* d_samples, int nsamples, const float* d_rot, int nrot,
+              float radius, float thresh, float factor, float* d_ao, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (!d_norm || !d_samples || !d_rot || !d_ao) return fail(c, TINA_ERR_INVALID, "tina_ssao: buffer is NULL");
+  if (nsamples < 1 || nrot < 1) return fail(c, TINA_ERR_INVALID, "tina_ssao: nsamples and nrot must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = flush_clear(c, s);
+  if (rc) return rc;
+  const long npix = (long)c->W * c->H;
+  hipLaunchKernelGGL(k_ssao, dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, make_cam(c),
+                     (const uint64_t*)c->keys, d_norm, d_samples, nsamples, d_rot, nrot, radius, thresh, factor, d_ao);
+  TINA_LAUNCH(c, "k_ssao");
+  return TINA_OK;
+}
+
+int tina_ssao_apply(float* d_img, int width, int height, const float* d_ao, int nrot, int box, void* stream) {
+  if (!d_img || !d_ao) return fail(nullptr, TINA_ERR_INVALID, "tina_ssao_apply: buffer is NULL");
+  if (width < 1 || height < 1 || nrot < 1) return fail(nullptr, TINA_ERR_INVALID, "tina_ssao_apply: bad size");
+  const long n = (long)width * height;
+  hipLaunchKernelGGL(k_ssao_apply, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_img, d_ao,
+                     width, height, nrot, box ? 1 : 0);
+  return hip_check(nullptr, hipGetLastError(), "k_ssao_apply");
+}
+
 int tina_shade(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,
                const tina_ibl* ibl, float* d_img, const float bg[3], uint32_t flags, void* stream) {
   if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");

@@ -1,12 +1,12 @@
 """Image post-processing on the [H, W, 3] f32 image: ToneMapping
 (tina/postp/tonemap.py:4-12, in-place ACES), Blooming (tina/postp/blooming.py),
-FXAA (tina/postp/fxaa.py) and the TAA Accumator (tina/util/accumator.py), each
-one call into libtina_hip.so."""
+FXAA (tina/postp/fxaa.py), SSAO (tina/postp/ssao.py) and the TAA Accumator
+(tina/util/accumator.py), each one or two calls into libtina_hip.so."""
 import ctypes
 
 from . import _abi
 
-__all__ = ['ToneMapping', 'Blooming', 'FXAA', 'Accumator']
+__all__ = ['ToneMapping', 'Blooming', 'FXAA', 'Accumator', 'SSAO']
 
 
 class ToneMapping:
@@ -115,3 +115,70 @@ class Accumator:
         self.count += 1
         _abi.check(_abi.load().tina_accumulate(_abi.ptr(self._t), _abi.ptr(t), int(t.numel()), int(self.count),
                                                _abi.stream_handle()))
+
+
+class SSAO:
+    """Screen-space ambient occlusion (tina/postp/ssao.py:5-96) from the depth
+    buffer and a normal buffer (a Field [W, H, 3] filled by NormalShader).
+
+    The nsamples hemisphere samples spherical(lerp(u, .01, 1), v) *
+    lerp(w ** 1.5, .01, 1) and the noise_size x noise_size tile of rotations
+    (cos t, sin t) are drawn on the host (NumPy, `seed`), where the reference
+    draws them with ti.random; seed_samples() draws a new set.  With taa the
+    reference takes nsamples // 4 fresh samples per pixel and skips the box
+    filter; here each frame draws a fresh set of nsamples // 4 samples and
+    rotations and skips the box filter."""
+
+    def __init__(self, res, norm, nsamples=64, thresh=0.0, radius=0.2, factor=1.0, noise_size=4, taa=False,
+                 seed=0):
+        import numpy as np
+        import torch
+        self.res = (int(res[0]), int(res[1]))
+        W, H = self.res
+        dev = torch.device('cuda', torch.cuda.current_device())
+        self.norm = norm
+        self.radius, self.thresh, self.factor = float(radius), float(thresh), float(factor)
+        self.taa = bool(taa)
+        self.nsamples = int(nsamples) if not self.taa else int(nsamples) // 4
+        self.noise_size = int(noise_size)
+        self.img = torch.zeros((H, W), dtype=torch.float32, device=dev)
+        self._rng = np.random.default_rng(seed)
+        self.seed_samples()
+
+    @staticmethod
+    def make_samples(rng, n):
+        """make_sample (ssao.py:50-55) for n samples, in binary32."""
+        import numpy as np
+        f32 = np.float32
+        u, v, w = (rng.random(n).astype(f32) for _ in range(3))
+        r = (f32(0.01) * (f32(1) - w ** f32(1.5)) + f32(1.0) * w ** f32(1.5)).astype(f32)
+        u = (f32(0.01) * (f32(1) - u) + f32(1.0) * u).astype(f32)
+        t = (v * f32(2 * np.pi)).astype(f32)
+        s = np.sqrt(np.maximum(f32(0), f32(1) - u * u)).astype(f32)
+        return (np.stack([s * np.cos(t), s * np.sin(t), u], axis=1).astype(f32) * r[:, None]).astype(f32)
+
+    def seed_samples(self):
+        """seed_samples (ssao.py:28-35): the sample set and the rotation tile."""
+        import numpy as np
+        import torch
+        dev = self.img.device
+        self.samples = torch.as_tensor(self.make_samples(self._rng, self.nsamples), device=dev).contiguous()
+        t = (self._rng.random((self.noise_size, self.noise_size)) * 2 * np.pi).astype(np.float32)
+        rot = np.stack([np.cos(t), np.sin(t)], axis=2).astype(np.float32)
+        self.rotations = torch.as_tensor(rot, device=dev).contiguous()
+
+    def render(self, engine):
+        """render (ssao.py:61-65): the occlusion of every pixel into self.img."""
+        if self.taa:
+            self.seed_samples()
+        n = self.norm.storage
+        _abi.check(engine._lib.tina_ssao(engine._ctx, _abi.ptr(n), _abi.ptr(self.samples), int(self.nsamples),
+                                         _abi.ptr(self.rotations), int(self.noise_size), ctypes.c_float(self.radius),
+                                         ctypes.c_float(self.thresh), ctypes.c_float(self.factor),
+                                         _abi.ptr(self.img), _abi.stream_handle()), engine._ctx)
+
+    def apply(self, image):
+        """apply (ssao.py:37-48): image *= 1 - (box-filtered) occlusion."""
+        t, W, H = _dims(image)
+        _abi.check(_abi.load().tina_ssao_apply(_abi.ptr(t), W, H, _abi.ptr(self.img), int(self.noise_size),
+                                               0 if self.taa else 1, _abi.stream_handle()))

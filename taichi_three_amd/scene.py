@@ -14,15 +14,15 @@ from .engine import Engine, MAX
 from .field import Field
 from .lighting import Lighting, SkyboxLighting
 from .matr import Diffuse
-from .postp import ToneMapping, Blooming, FXAA, Accumator
+from .postp import ToneMapping, Blooming, FXAA, Accumator, SSAO
 from .raster import TriangleRaster
 from .particle import ParticleRaster
 from .wireframe import WireframeRaster
-from .shader import Shader, ShaderGroup
+from .shader import Shader, ShaderGroup, NormalShader
 
 __all__ = ['Scene']
 
-_UNSUPPORTED = ('ssr', 'ssao')
+_UNSUPPORTED = ('ssr',)
 
 
 class _Obj:
@@ -38,7 +38,7 @@ class Scene:
     def __init__(self, res_x=512, res_y=None, **options):
         for k in _UNSUPPORTED:
             if options.get(k, False):
-                raise NotImplementedError(f'Scene({k}=True): post-processing passes are not provided by taichi_three_amd')
+                raise NotImplementedError(f'Scene({k}=True): screen-space reflections are not provided by taichi_three_amd')
         self.options = options
         self.engine = Engine(res_x, res_y, maxfaces=options.get('maxfaces', MAX))
         self.res = self.engine.res
@@ -76,8 +76,15 @@ class Scene:
         # temporal anti-aliasing: a jittered pixel centre per frame, averaged (raster.py:83-84, 173-174, 206-214)
         self.taa = bool(options.get('taa', False))
         self.accum = Accumator(self.res) if self.taa else None
-        if self.blooming is not None and self.neural is not None:
-            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming')
+        # SSAO reads a normal buffer that a pre-shader fills per object (raster.py:52-69)
+        self.ssao = None
+        if options.get('ssao', False):
+            self.norm_buffer = Field(torch.zeros((H, W, 3), dtype=torch.float32, device=self.engine.device))
+            self.norm_shader = NormalShader(self.norm_buffer)
+            self.pre_shaders.append(self.norm_shader)
+            self.ssao = SSAO(self.res, self.norm_buffer, taa=self.taa)
+        if (self.blooming is not None or self.ssao is not None) and self.neural is not None:
+            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming or SSAO')
 
     def _ensure_material_shader(self, material):
         if material in self.materials:
@@ -146,15 +153,21 @@ class Scene:
         mark('end')
 
     def _shade_all(self, order):
-        # tone mapping is fused into the shade kernels unless blooming has to see the linear image
-        tm = _abi.TONEMAP if self.tonemap and self.blooming is None else 0
-        if self.blooming is not None and self.neural is not None:
-            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming')
+        # tone mapping is fused into the shade kernels unless SSAO or blooming has to see the linear image
+        linear = self.blooming is not None or self.ssao is not None
+        tm = _abi.TONEMAP if self.tonemap and not linear else 0
+        if linear and self.neural is not None:
+            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming or SSAO')
         self._shade_objects(order, tm)
+        if self.ssao is not None:               # raster.py:189-191, then the background (:197-198)
+            self.ssao.render(self.engine)
+            self.ssao.apply(self.image)
+            if isinstance(self.lighting, SkyboxLighting):
+                self._fill_background(0)        # the sky is drawn after the occlusion, over empty pixels
         if self.blooming is not None:
             self.blooming.apply(self.image)
-            if self.tonemap:
-                self.tonemap.apply(self.image)
+        if self.tonemap and linear:
+            self.tonemap.apply(self.image)
         if self.fxaa is not None:
             self.fxaa.apply(self.image)
 

@@ -7,7 +7,7 @@ ShaderGroup collects it with post shaders such as ProbeShader, which read the
 resolved id buffer after the launch.
 """
 
-__all__ = ['IShader', 'Shader', 'ShaderGroup', 'BackgroundShader']
+__all__ = ['IShader', 'Shader', 'ShaderGroup', 'BackgroundShader', 'NormalShader']
 
 
 class IShader:
@@ -58,3 +58,24 @@ class BackgroundShader(IShader):
     def __init__(self, img, lighting):
         super().__init__(img)
         self.lighting = lighting
+
+
+class NormalShader(IShader):
+    """img[P] = the shading normal (shader.py:43-46), for the pixels each object
+    won; runs after that object's colour pass (tina_normals).  Wire pixels are
+    left as they are (wires have no normal)."""
+
+    def post_shade(self, raster, id_base, nfaces):
+        from . import _abi
+        if nfaces == 0 or not hasattr(raster, 'mesh_struct'):
+            return
+        flags = 0
+        if type(raster).__name__ == 'ParticleRaster':
+            flags = _abi.PARTICLES
+        elif getattr(raster, 'smoothing', False):
+            flags = _abi.SMOOTH
+        eng = raster.engine
+        mesh = raster.mesh_struct()
+        import ctypes
+        _abi.check(eng._lib.tina_normals(eng._ctx, ctypes.byref(mesh), flags, _abi.ptr(self.img.storage),
+                                         _abi.stream_handle()), eng._ctx)
