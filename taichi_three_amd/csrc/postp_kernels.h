@@ -102,6 +102,85 @@ k_bloom_blur(const float* __restrict__ src, float* __restrict__ dst, int hw, int
   }
 }
 
+// The same blur with the taps read from LDS.  Along x a workgroup stages 256
+// pixels of one row plus `radius` on each side; along y it stages a tile of TX
+// columns by 32 rows plus `radius` rows above and below (TX = 32 or 16, chosen
+// so the tile fits 64 KB).  Staged texels are the clamped ones the direct
+// kernel reads, kept as three planes (no bank conflicts), and every output sums
+// its taps in the same order: identical results, one HBM/L2 read per staged
+// texel instead of 2 radius + 1 per output.
+constexpr int kBlurRow = 256;
+constexpr int kBlurRows = 32;
+
+__global__ void __launch_bounds__(kBlock)
+k_bloom_blur_x(const float* __restrict__ src, float* __restrict__ dst, int hw, int hh, const float* __restrict__ gwei,
+               int radius) {
+  extern __shared__ uint32_t lds_words[];
+  float* lds = (float*)lds_words;
+  const int n = kBlurRow + 2 * radius;
+  const int y = blockIdx.y, x0 = blockIdx.x * kBlurRow, t = threadIdx.x;
+  for (int i = t; i < n; i += kBlock) {
+    const int xs = min(max(x0 - radius + i, 0), hw - 1);
+    const float* p = src + ((long)y * hw + xs) * 3;
+    lds[i] = p[0];
+    lds[n + i] = p[1];
+    lds[2 * n + i] = p[2];
+  }
+  __syncthreads();
+  const int x = x0 + t;
+  if (x >= hw) return;
+  const int i = t + radius;
+  const float g0 = gwei[0];
+  V3 res = v3(lds[i] * g0, lds[n + i] * g0, lds[2 * n + i] * g0);
+  for (int k = 1; k <= radius; k++) {
+    const V3 val = v3(lds[i - k] + lds[i + k], lds[n + i - k] + lds[n + i + k], lds[2 * n + i - k] + lds[2 * n + i + k]);
+    const float g = gwei[k];
+    res = vadd(res, v3(val.x * g, val.y * g, val.z * g));
+  }
+  float* o = dst + ((long)y * hw + x) * 3;
+  o[0] = res.x;
+  o[1] = res.y;
+  o[2] = res.z;
+}
+
+template <int TX>
+__global__ void __launch_bounds__(kBlock)
+k_bloom_blur_y(const float* __restrict__ src, float* __restrict__ dst, int hw, int hh, const float* __restrict__ gwei,
+               int radius) {
+  extern __shared__ uint32_t lds_words[];
+  float* lds = (float*)lds_words;
+  const int rows = kBlurRows + 2 * radius, n = rows * TX;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * kBlurRows, t = threadIdx.x;
+  for (int idx = t; idx < n; idx += kBlock) {
+    const int r = idx / TX, c = idx - r * TX;
+    const int xs = min(x0 + c, hw - 1), ys = min(max(y0 - radius + r, 0), hh - 1);
+    const float* p = src + ((long)ys * hw + xs) * 3;
+    lds[idx] = p[0];
+    lds[n + idx] = p[1];
+    lds[2 * n + idx] = p[2];
+  }
+  __syncthreads();
+  const int c = t % TX, x = x0 + c;
+  if (x >= hw) return;
+  const float g0 = gwei[0];
+  for (int ry = t / TX; ry < kBlurRows; ry += kBlock / TX) {
+    const int y = y0 + ry;
+    if (y >= hh) break;
+    const int i = (ry + radius) * TX + c;
+    V3 res = v3(lds[i] * g0, lds[n + i] * g0, lds[2 * n + i] * g0);
+    for (int k = 1; k <= radius; k++) {
+      const int a = i - k * TX, b = i + k * TX;
+      const V3 val = v3(lds[a] + lds[b], lds[n + a] + lds[n + b], lds[2 * n + a] + lds[2 * n + b]);
+      const float g = gwei[k];
+      res = vadd(res, v3(val.x * g, val.y * g, val.z * g));
+    }
+    float* o = dst + ((long)y * hw + x) * 3;
+    o[0] = res.x;
+    o[1] = res.y;
+    o[2] = res.z;
+  }
+}
+
 // image[I] += bilerp(half, I / 2)
 __global__ void __launch_bounds__(kBlock)
 k_bloom_add(float* __restrict__ img, int W, int H, const float* __restrict__ half, int hw, int hh) {

@@ -556,10 +556,25 @@ int tina_bloom(float* d_img, int width, int height, float thresh, float factor, 
   const long nh = (long)hw * hh, n = (long)width * height;
   hipLaunchKernelGGL(k_bloom_down, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_img, width, d_half, hw, hh,
                      thresh, scale, factor);
-  hipLaunchKernelGGL(k_bloom_blur, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_half, d_tmp, hw, hh,
-                     d_gwei, radius, 0);
-  hipLaunchKernelGGL(k_bloom_blur, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_tmp, d_half, hw, hh,
-                     d_gwei, radius, 1);
+  // the LDS-staged passes when their tiles fit 64 KB (radius up to ~1300 along x, ~150 along y)
+  const size_t lds_x = (size_t)(kBlurRow + 2 * radius) * 12;
+  const size_t lds_y32 = (size_t)(kBlurRows + 2 * radius) * 32 * 12, lds_y16 = lds_y32 / 2;
+  if (lds_x <= 65536)
+    hipLaunchKernelGGL(k_bloom_blur_x, dim3((hw + kBlurRow - 1) / kBlurRow, hh), dim3(kBlock), lds_x, s, d_half,
+                       d_tmp, hw, hh, d_gwei, radius);
+  else
+    hipLaunchKernelGGL(k_bloom_blur, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_half, d_tmp, hw, hh,
+                       d_gwei, radius, 0);
+  const unsigned gy = (unsigned)((hh + kBlurRows - 1) / kBlurRows);
+  if (lds_y32 <= 65536)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bloom_blur_y<32>), dim3((hw + 31) / 32, gy), dim3(kBlock), lds_y32, s, d_tmp,
+                       d_half, hw, hh, d_gwei, radius);
+  else if (lds_y16 <= 65536)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bloom_blur_y<16>), dim3((hw + 15) / 16, gy), dim3(kBlock), lds_y16, s, d_tmp,
+                       d_half, hw, hh, d_gwei, radius);
+  else
+    hipLaunchKernelGGL(k_bloom_blur, dim3(grid_for(nh, kBlock, 4096)), dim3(kBlock), 0, s, d_tmp, d_half, hw, hh,
+                       d_gwei, radius, 1);
   hipLaunchKernelGGL(k_bloom_add, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, s, d_img, width, height, d_half,
                      hw, hh);
   return hip_check(nullptr, hipGetLastError(), "k_bloom");

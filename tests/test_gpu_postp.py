@@ -15,15 +15,21 @@ def _field(arr):
     return Field(torch.as_tensor(np.ascontiguousarray(arr), device='cuda'))
 
 
-@pytest.mark.parametrize('W,H', [(97, 64), (256, 144)])
-def test_bloom_and_fxaa_match_the_oracle(gpu, W, H):
+@pytest.mark.parametrize('W,H,radius', [(97, 64, None), (256, 144, None), (1920, 1080, None), (601, 333, 90),
+                                         (500, 300, 200)])
+def test_bloom_and_fxaa_match_the_oracle(gpu, W, H, radius):
+    """radius None is the reference's min(res) // 16 (67 at 1920x1080); 90 takes the
+    16-column vertical tile, 200 the direct (unstaged) vertical pass."""
     import torch
     import taichi_three_amd as tina
-    print(f'case: bloom + fxaa {W}x{H}', flush=True)
+    print(f'case: bloom + fxaa {W}x{H} radius {radius}', flush=True)
     rng = np.random.default_rng(W + H)
     img = (rng.random((H, W, 3)) ** 3 * 4).astype(f32)
     img[H // 3:H // 2, W // 4:W // 2] = 6.0  # a bright block with hard edges
     bl = tina.Blooming((W, H))
+    if radius is not None:
+        bl.radius = radius
+        bl.init_gwei()
     f = _field(img)
     bl.apply(f)
     torch.cuda.synchronize()
