@@ -15,7 +15,8 @@ import numpy as np
 from .engine import MAX
 
 __all__ = ['SimpleMesh', 'PrimitiveMesh', 'MeshFlatNormal', 'MeshSmoothNormal', 'ConnectiveMesh',
-           'MeshTransform', 'MeshEditBase', 'MeshModel', 'MeshToWire']
+           'MeshTransform', 'MeshEditBase', 'MeshModel', 'MeshToWire', 'MeshFlipCulling', 'MeshNoCulling',
+           'MeshFlipNormal', 'MeshGrid', 'export_simple_mesh', 'simple_mesh_to_connective', 'export_connective_mesh']
 
 
 def _torch():
@@ -454,3 +455,162 @@ class MeshToWire(MeshEditBase):
         out['nfaces'] = int(fb['nfaces']) * k
         out['attrs'] = {}
         return out
+
+
+def _flip_corners(a):
+    return None if a is None else a.flip(1).contiguous()
+
+
+class MeshFlipCulling(MeshEditBase):
+    """Every face's corners in reverse order, so culling keeps the other side
+    (tina/mesh/cull.py:5-26)."""
+
+    def face_buffers(self):
+        fb = dict(self.mesh.face_buffers())
+        for key in ('verts', 'norms', 'coors', 'colors'):
+            fb[key] = _flip_corners(fb.get(key))
+        return fb
+
+
+class MeshNoCulling(MeshEditBase):
+    """Each face twice: face 2n as it is, face 2n + 1 with its corners reversed
+    and its normals negated, so both sides survive culling (cull.py:29-57)."""
+
+    def get_nfaces(self):
+        return self.mesh.get_nfaces() * 2
+
+    def face_buffers(self):
+        torch = _torch()
+        fb = dict(self.mesh.face_buffers())
+
+        def pair(a, negate=False):
+            if a is None:
+                return None
+            b = a.flip(1)
+            if negate:
+                b = -b
+            return torch.stack([a, b], dim=1).reshape(-1, *a.shape[1:]).contiguous()
+        for key in ('verts', 'coors', 'colors'):
+            fb[key] = pair(fb.get(key))
+        fb['norms'] = pair(fb.get('norms'), negate=True)
+        fb['attrs'] = {k: v.repeat_interleave(2, dim=0).contiguous() for k, v in fb.get('attrs', {}).items()}
+        fb['nfaces'] = int(fb['nfaces']) * 2
+        return fb
+
+
+class MeshFlipNormal(MeshEditBase):
+    """Normals negated (cull.py:60-66)."""
+
+    def face_buffers(self):
+        fb = dict(self.mesh.face_buffers())
+        if fb.get('norms') is not None:
+            fb['norms'] = (-fb['norms']).contiguous()
+        return fb
+
+
+class MeshGrid:
+    """A res.x x res.y grid of vertices (tina/mesh/grid.py:5-74): pos, nrm and
+    tex are Fields indexed [i, j]; pos starts as the plane (u * 2 - 1, v * 2 - 1, 0)
+    with (u, v) = (i, j) / (res - 1), tex = (u, v).  pre_compute() sets each
+    normal to normalize(dx x dy) of the central differences (clamped at the
+    border).  Cell (i, j) gives the triangles (a, b, c) and (a, c, d) of its
+    corners a = [i, j], b = [i + 1, j], c = [i + 1, j + 1], d = [i, j + 1], or the
+    quad (a, b, c, d) with as_quad (for MeshToWire)."""
+
+    def __init__(self, res, as_quad=False, device=None):
+        from .field import Field
+        torch = _torch()
+        if isinstance(res, int):
+            res = (res, res)
+        self.res = (int(res[0]), int(res[1]))
+        self.as_quad = as_quad
+        dev = device or torch.device('cuda')
+        rx, ry = self.res
+        i = torch.arange(rx, dtype=torch.float32, device=dev)[:, None].expand(rx, ry)
+        j = torch.arange(ry, dtype=torch.float32, device=dev)[None, :].expand(rx, ry)
+        u, v = i / float(rx - 1), j / float(ry - 1)
+        self.tex = Field(torch.stack([u, v], dim=2).contiguous(), screen=False)
+        self.pos = Field(torch.stack([u * 2 - 1, v * 2 - 1, torch.zeros_like(u)], dim=2).contiguous(), screen=False)
+        self.nrm = Field(torch.zeros((rx, ry, 3), dtype=torch.float32, device=dev), screen=False)
+        self.version = 0
+
+    def get_npolygon(self):
+        return 4 if self.as_quad else 3
+
+    def get_nfaces(self):
+        n = (self.res[0] - 1) * (self.res[1] - 1)
+        return n if self.as_quad else n * 2
+
+    def pre_compute(self):
+        """nrm[i, j] = normalized(dx.cross(dy)) (grid.py:26-35)."""
+        torch = _torch()
+        p = self.pos.storage
+        rx, ry = self.res
+        ii = torch.arange(rx, device=p.device)
+        jj = torch.arange(ry, device=p.device)
+        i1, i2 = (ii + 1).clamp(max=rx - 1), (ii - 1).clamp(min=0)
+        j1, j2 = (jj + 1).clamp(max=ry - 1), (jj - 1).clamp(min=0)
+        dy = p[:, j1] - p[:, j2]
+        dx = p[i1, :] - p[i2, :]
+        n = torch.stack([dx[..., 1] * dy[..., 2] - dx[..., 2] * dy[..., 1],
+                         dx[..., 2] * dy[..., 0] - dx[..., 0] * dy[..., 2],
+                         dx[..., 0] * dy[..., 1] - dx[..., 1] * dy[..., 0]], dim=2)
+        inv = 1.0 / torch.sqrt((n[..., 0] * n[..., 0] + n[..., 1] * n[..., 1]) + n[..., 2] * n[..., 2])
+        self.nrm.storage.copy_(n * inv[..., None])
+
+    def _faces(self, prop):
+        a, b = prop[:-1, :-1], prop[1:, :-1]
+        c, d = prop[1:, 1:], prop[:-1, 1:]
+        if self.as_quad:
+            f = [a, b, c, d]
+            return _torch().stack(f, dim=2).reshape(-1, 4, prop.shape[-1]).contiguous()
+        t0 = _torch().stack([a, b, c], dim=2)
+        t1 = _torch().stack([a, c, d], dim=2)
+        return _torch().stack([t0, t1], dim=2).reshape(-1, 3, prop.shape[-1]).contiguous()
+
+    def face_buffers(self):
+        # cell m = i * (res.y - 1) + j, its two triangles 2m and 2m + 1 (grid.py:46-60)
+        return dict(verts=self._faces(self.pos.storage), norms=self._faces(self.nrm.storage),
+                    coors=self._faces(self.tex.storage), colors=None, attrs={}, nfaces=self.get_nfaces())
+
+
+def export_simple_mesh(mesh):
+    """{'fv': [n, npolygon, 3], 'fn': ..., 'ft': [n, npolygon, 2]} float32 (tina/mesh/export.py:66-85)."""
+    if hasattr(mesh, 'pre_compute'):
+        mesh.pre_compute()
+    fb = mesh.face_buffers()
+    out = {'fv': fb['verts'].detach().cpu().numpy().astype(np.float32)}
+    if fb.get('norms') is not None:
+        out['fn'] = fb['norms'].detach().cpu().numpy().astype(np.float32)
+    if fb.get('coors') is not None:
+        out['ft'] = fb['coors'].detach().cpu().numpy().astype(np.float32)
+    return out
+
+
+def simple_mesh_to_connective(obj):
+    """One vertex per face corner: 'f' [n, npolygon] = arange, 'v' [n * npolygon, 3]
+    (export.py:88-98; normals and texcoords under 'fn' / 'ft' become 'vn' / 'vt')."""
+    fv = np.asarray(obj['fv'])
+    nf, k = fv.shape[:2]
+    ret = {'f': np.arange(nf * k).reshape(nf, k), 'v': fv.reshape(nf * k, 3)}
+    if 'fn' in obj:
+        ret['vn'] = np.asarray(obj['fn']).reshape(nf * k, 3)
+    if 'ft' in obj:
+        ret['vt'] = np.asarray(obj['ft']).reshape(nf * k, -1)
+    return ret
+
+
+def export_connective_mesh(mesh):
+    """An indexed mesh's own arrays (ConnectiveMesh, MeshModel), else the face soup
+    with one vertex per corner (export.py:101-126)."""
+    inner = mesh
+    while isinstance(inner, MeshEditBase) and not hasattr(inner, 'faces'):
+        inner = inner.mesh
+    if hasattr(inner, 'faces') and hasattr(inner, 'verts_v') and inner is mesh:
+        f = inner.faces.cpu().numpy()
+        ret = {'f': f[..., 0] if f.ndim == 3 else f, 'v': inner.verts_v.cpu().numpy()}
+        for src, dst in (('norms_v', 'vn'), ('coors_v', 'vt')):
+            if getattr(inner, src, None) is not None:
+                ret[dst] = getattr(inner, src).cpu().numpy()
+        return ret
+    return simple_mesh_to_connective(export_simple_mesh(mesh))
