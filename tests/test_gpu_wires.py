@@ -166,3 +166,58 @@ def test_triangles_wires_particles_in_one_scene(gpu, wire_first):
     nan = np.isnan(img)
     assert np.array_equal(nan, np.isnan(exp))
     assert np.abs(img[~nan] - exp[~nan]).max() <= COLOR_TOL_F32
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_random_particle_and_wire_scenes(gpu, seed):
+    """Random screens, perspective or orthographic cameras, sub-pixel biases and
+    clipping choices with a particle cloud and a wire soup in one scene: ids and
+    depth bit for bit against the oracle, particle colour within 1e-5 of binary32."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import _abi
+    from taichi_three_amd.matr import lower_material
+    rng = np.random.default_rng(500 + seed)
+    W, H = int(rng.integers(48, 1300)), int(rng.integers(48, 800))
+    npar, nw = int(rng.integers(500, 30000)), int(rng.integers(100, 8000))
+    spread = rng.uniform(0.5, 2.0)
+    pos = rng.uniform(-spread, spread, (npar, 3)).astype(np.float32)
+    rad = np.exp(rng.uniform(np.log(0.002), np.log(0.15), npar)).astype(np.float32)
+    col = rng.uniform(0.1, 1, (npar, 3)).astype(np.float32)
+    wires = rng.uniform(-spread, spread, (nw, 2, 3)).astype(np.float32)
+    d = rng.normal(size=3)
+    view = tina.lookat(pos=tuple(rng.uniform(-0.3, 0.3, 3)), back=tuple(rng.uniform(1.0, 4.0) * d / np.linalg.norm(d)))
+    ortho = bool(rng.integers(0, 2))
+    proj = tina.orthogonal(spread * 1.5, W / H) if ortho else tina.perspective(float(rng.uniform(30, 100)), W / H, 0.05, 50)
+    clipping = seed % 2 == 1
+    bias = rng.uniform(0, 1, 2).astype(np.float32)
+    print(f'case: random particles+wires seed={seed} {W}x{H} npar={npar} nwires={nw} ortho={ortho} '
+          f'clipping={clipping}', flush=True)
+    scene = tina.Scene(W, H, maxpars=npar, clipping=clipping)
+    pars = tina.SimpleParticles(npar)
+    pars.set_particles(pos)
+    pars.set_particle_radii(rad)
+    pars.set_particle_colors(col)
+    mesh = tina.SimpleMesh(nw, npolygon=2)
+    mesh.set_face_verts(wires)
+    mat = tina.Diffuse()
+    scene.add_object(pars, mat)
+    scene.add_object(mesh)
+    scene.set_camera(view, proj)
+    scene.engine.bias = bias
+    scene.engine._push_camera()
+    ids, depth, img = _run(scene)
+    eng = scene.engine
+    cam = orc.camera(eng.W2V, eng.V2W, eng.bias)
+    fl = _abi.CLIP if clipping else 0
+    od, oi = orc.raster_particles(W, H, cam, pos, rad, flags=fl)
+    od, oi = orc.raster_wires(W, H, cam, wires, id_base=npar, flags=fl, depth=od, occup=oi)
+    assert np.array_equal(ids, oi), f'{(ids != oi).sum()} id mismatches'
+    assert np.array_equal(depth, od)
+    lo = lower_material(mat, dict(attrs={}))
+    p32 = orc.shade(W, H, cam, orc.particle_mesh(pos, rad, col), lo.struct, scene.lighting.to_struct(), oi,
+                    _abi.PARTICLES | _abi.VCOLOR | _abi.BG | _abi.TONEMAP)
+    exp = np.where(((oi >= npar))[..., None], orc.aces(LINE[None])[0], p32)
+    nan = np.isnan(img)
+    assert np.array_equal(nan, np.isnan(exp))
+    assert np.abs(img[~nan] - exp[~nan]).max() <= COLOR_TOL_F32
