@@ -397,6 +397,13 @@ def main():
         out['batch'] = batched_views(max(5, args.steps // 4))
     out['postp'] = post_passes(max(10, args.steps // 2))
     out['other_rasters'] = other_rasters(max(10, args.steps // 2))
+    # SSAO on the same frame: the shade stage then also writes the normal buffer,
+    # runs the 64-sample occlusion, the 4x4 box and ACES as a separate pass
+    scene_ao, _ = wl.build_scene(w, ssao=True)
+    ao_ms = stage_times(scene_ao, max(10, args.steps // 2))
+    out['ssao'] = {'shade_ms': round(ao_ms['shade'], 4), 'extra_ms': round(ao_ms['shade'] - stage_ms['shade'], 4),
+                   'nsamples': scene_ao.ssao.nsamples}
+    del scene_ao
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(w)
         out['speedup_vs_cpu'] = round(fps / out['cpu_baseline']['value'], 1)

@@ -306,7 +306,10 @@ k_ssao(const Cam cam, const uint64_t* __restrict__ keys, const float* __restrict
     const float* r = rot + 2 * ((long)(x % nrot) * nrot + (y % nrot));
     const float rx = r[0], ry = r[1];
     float occ = 0.0f;
-    for (int i = 0; i < nsamples; i++) {
+    // a zero normal (no object, or a wire) makes the tangent frame NaN and every
+    // sample fails its screen test, so the loop would add nothing: skip it
+    const bool frame = !(normal.x == 0.0f && normal.y == 0.0f && normal.z == 0.0f);
+    for (int i = 0; frame && i < nsamples; i++) {
       const float* sp = samples + 3 * i;
       const float sx = (rx * sp[0]) + (ry * sp[1]), sy = (-rx * sp[0]) + (ry * sp[1]), sz = sp[2];
       V3 smp = v3(((tan.x * sx) + (bitan.x * sy)) + (normal.x * sz), ((tan.y * sx) + (bitan.y * sy)) + (normal.y * sz),
