@@ -361,4 +361,33 @@ k_ssao_apply(float* __restrict__ img, const float* __restrict__ ao, int W, int H
   }
 }
 
+// k_ssao_apply's box form with the occlusion staged in LDS: a 16 x 16 block of
+// pixels reads its (16 + nrot - 1)^2 window once (clamped reads, as above) and
+// sums each box in the same order.  nrot <= 16.
+constexpr int kAoTile = 16;
+
+__global__ void __launch_bounds__(kBlock)
+k_ssao_apply_box(float* __restrict__ img, const float* __restrict__ ao, int W, int H, int nrot) {
+  __shared__ float s_ao[(kAoTile + 15) * (kAoTile + 15)];
+  const int offs = nrot / 2, side = kAoTile + nrot - 1;
+  const int x0 = blockIdx.x * kAoTile, y0 = blockIdx.y * kAoTile, t = threadIdx.x;
+  for (int i = t; i < side * side; i += kBlock) {
+    const int r = i / side, c = i - r * side;
+    const int xx = min(max(x0 + c - offs, 0), W - 1), yy = min(max(y0 + r - offs, 0), H - 1);
+    s_ao[i] = ao[(long)yy * W + xx];
+  }
+  __syncthreads();
+  const int tx = t % kAoTile, ty = t / kAoTile;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= W || y >= H) return;
+  float r = 0.0f;
+  for (int k = 0; k < nrot; k++)
+    for (int l = 0; l < nrot; l++) r = r + s_ao[(ty + l) * side + tx + k];
+  const float m = 1.0f - r / (float)(nrot * nrot);
+  float* p = img + 3 * ((long)y * W + x);
+  p[0] = p[0] * m;
+  p[1] = p[1] * m;
+  p[2] = p[2] * m;
+}
+
 }  // namespace tina

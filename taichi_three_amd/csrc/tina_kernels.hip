@@ -493,8 +493,12 @@ int tina_ssao_apply(float* d_img, int width, int height, const float* d_ao, int 
   if (!d_img || !d_ao) return fail(nullptr, TINA_ERR_INVALID, "tina_ssao_apply: buffer is NULL");
   if (width < 1 || height < 1 || nrot < 1) return fail(nullptr, TINA_ERR_INVALID, "tina_ssao_apply: bad size");
   const long n = (long)width * height;
-  hipLaunchKernelGGL(k_ssao_apply, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_img, d_ao,
-                     width, height, nrot, box ? 1 : 0);
+  if (box && nrot <= 16)
+    hipLaunchKernelGGL(k_ssao_apply_box, dim3((width + kAoTile - 1) / kAoTile, (height + kAoTile - 1) / kAoTile),
+                       dim3(kBlock), 0, (hipStream_t)stream, d_img, d_ao, width, height, nrot);
+  else
+    hipLaunchKernelGGL(k_ssao_apply, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_img,
+                       d_ao, width, height, nrot, box ? 1 : 0);
   return hip_check(nullptr, hipGetLastError(), "k_ssao_apply");
 }
 
