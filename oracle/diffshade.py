@@ -1,5 +1,5 @@
 """NumPy float64 restatement of the differentiable shading pass -- TEST
-INFRASTRUCTURE ONLY (imported by tests/ as the checker, never by the product).
+INFRASTRUCTURE ONLY (imported by tests/ as a reference, never by the product).
 
 The shading is Tina's PBR material (tina/matr/material.py:322-362, 387-396,
 701-706) under point lights (tina/core/lighting.py:84-98) with every material

@@ -1,6 +1,6 @@
 """NumPy restatement of the learned shading surrogate -- TEST INFRASTRUCTURE ONLY.
 
-Imported by tests/ and bench.py's checks, always as the checker; the product
+Imported by tests/ and bench.py's checks, always as the reference of a comparison; the product
 never imports it.  It restates, in float64 unless stated otherwise:
 
   features(n, V, L, r, m, albedo)   the 12 network inputs

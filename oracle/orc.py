@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (liborc.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker.  The product package never
+cpu_baseline leg of bench.py, always as the reference of a comparison.  The product package never
 imports it.  The restatement it wraps is described in tina_oracle.h; what pins
 each function is listed in oracle/README.md.
 """

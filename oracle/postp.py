@@ -1,6 +1,6 @@
 """NumPy restatement of Tina's image post-processing -- TEST INFRASTRUCTURE ONLY.
 
-Imported by tests/ as the checker; the product never imports it.  Binary32
+Imported by tests/ as the reference they compare with; the product never imports it.  Binary32
 throughout, with the reference's grouping:
 
   bloom(img, thresh, factor, scale, gwei, radius)   Blooming.apply

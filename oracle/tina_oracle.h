@@ -1,7 +1,7 @@
 /*
  * tina_oracle.h -- CPU restatement of Tina's rasterise-and-shade path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only
+ * TEST INFRASTRUCTURE ONLY.  This library is what the HIP path is compared with: only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
  * product (taichi_three_amd + libtina_hip.so) never links or calls it.
  *
