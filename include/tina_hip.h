@@ -211,6 +211,16 @@ int tina_paint_ids(tina_ctx *ctx, uint32_t id_base, int n, const float color[3],
  * (interpolated vertex normals) or TINA_PARTICLES (the sphere normal). */
 int tina_normals(tina_ctx *ctx, const tina_mesh *mesh, uint32_t flags, float *d_norm, void *stream);
 
+/* The auxiliary shaders of tina/core/shader.py:21-109: for every pixel whose
+ * winning id lies in [mesh->id_base, +mesh->nfaces) writes `channels` (1..3)
+ * floats per pixel into d_out ([H][W][channels]): mode 0 NormalShader,
+ * 1 ViewNormalShader, 2 TexcoordShader, 3 ColorShader, 4 PositionShader,
+ * 5 DepthShader, 6 ConstShader (value), 7 ViewdirShader, 8 SimpleShader,
+ * 9 ChessboardShader (size).  flags: TINA_SMOOTH, TINA_TEX, TINA_VCOLOR or
+ * TINA_PARTICLES select the fragment as tina_shade does. */
+int tina_gbuffer(tina_ctx *ctx, const tina_mesh *mesh, uint32_t flags, int mode, int channels,
+                 const float value[3], float size, float *d_out, void *stream);
+
 /* SSAO.render (tina/postp/ssao.py:61-96): ambient occlusion per pixel from the
  * depth buffer and d_norm, with nsamples hemisphere samples d_samples [n][3]
  * turned by the nrot x nrot tile d_rot [nrot][nrot][2] (cos, sin); writes

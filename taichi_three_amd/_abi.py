@@ -126,6 +126,7 @@ EXPORTS = [
     ('tina_raster_wires', _i32, [_vp, _vp, _i32, _u32, _u32, _vp]),
     ('tina_paint_ids', _i32, [_vp, _u32, _i32, _vp, _vp, _u32, _vp, _vp]),
     ('tina_normals', _i32, [_vp, ctypes.POINTER(Mesh), _u32, _vp, _vp]),
+    ('tina_gbuffer', _i32, [_vp, ctypes.POINTER(Mesh), _u32, _i32, _i32, _vp, ctypes.c_float, _vp, _vp]),
     ('tina_ssao', _i32, [_vp, _vp, _vp, _i32, _vp, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp]),
     ('tina_ssao_apply', _i32, [_vp, _i32, _i32, _vp, _i32, _i32, _vp]),
     ('tina_shade', _i32, [_vp, ctypes.POINTER(Mesh), ctypes.POINTER(Material), ctypes.POINTER(Lights),

@@ -261,25 +261,54 @@ k_accumulate(float* __restrict__ acc, const float* __restrict__ src, long n, flo
 
 // ---- SSAO (tina/postp/ssao.py) and the normal buffer it reads ---------------
 
-// NormalShader.shade_color (tina/core/shader.py:43-46): the shading normal of
-// every pixel won by an id of the mesh (particles: their sphere normal).
+// The per-pixel buffers of Tina's auxiliary shaders (tina/core/shader.py:21-85,
+// 96-109), written for every pixel won by an id of the mesh (particles: from
+// their sphere fragment), `channels` floats per pixel:
+//   0 NormalShader      normal                   5 DepthShader     engine.depth[P]
+//   1 ViewNormalShader  normalize(mapply_dir(W2V, normal))
+//   2 TexcoordShader    texcoord                 6 ConstShader     value
+//   3 ColorShader       color                    7 ViewdirShader   viewdir * 0.5 + 0.5
+//   4 PositionShader    pos                      8 SimpleShader    |normal . viewdir|
+//   9 ChessboardShader  lerp((p // size).sum() % 2, 0.4, 0.9), p = P + bias
 template <bool PART>
 __global__ void __launch_bounds__(kBlock)
-k_normals(const ShadeArgs a, const uint64_t* __restrict__ keys, float* __restrict__ norm) {
+k_gbuffer(const ShadeArgs a, const uint64_t* __restrict__ keys, float* __restrict__ out, int mode, int channels,
+          float c0, float c1, float c2, float size) {
   const long npix = (long)a.cam.W * a.cam.H;
   for (long P = (long)blockIdx.x * kBlock + threadIdx.x; P < npix; P += (long)gridDim.x * kBlock) {
     const uint64_t k = keys[P];
     if (key_depth(k) >= TINA_MAXDEPTH) continue;
     const uint32_t f = (uint32_t)k - a.mesh.id_base;
     if (f >= (uint32_t)a.mesh.nfaces) continue;
-    Frag fr;
     const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
-    if (PART) particle_fragment(a, f, px, py, fr);
-    else fragment(a, f, px, py, fr);
-    float* o = norm + 3 * P;
-    o[0] = fr.nrm.x;
-    o[1] = fr.nrm.y;
-    o[2] = fr.nrm.z;
+    V3 v = v3(c0, c1, c2);
+    if (mode == 5) {
+      v = vs((float)key_depth(k));
+    } else if (mode == 9) {
+      const float ux = floorf(((float)px + a.cam.bias[0]) / size), uy = floorf(((float)py + a.cam.bias[1]) / size);
+      const float fac = fmodf(ux + uy, 2.0f);
+      v = vs(0.4f * (1.0f - fac) + 0.9f * fac);
+    } else if (mode != 6) {
+      Frag fr;
+      if (PART) particle_fragment(a, f, px, py, fr);
+      else fragment(a, f, px, py, fr);
+      switch (mode) {
+        case 0: v = fr.nrm; break;
+        case 1: v = vnormalized(mapply_dir(a.cam.W2V, fr.nrm)); break;
+        case 2: v = fr.tc; break;
+        case 3: v = fr.color; break;
+        case 4: v = fr.pos; break;
+        default: {
+          const V3 viewdir = vneg(view_ray(a.cam, px, py));
+          if (mode == 7) v = v3(viewdir.x * 0.5f + 0.5f, viewdir.y * 0.5f + 0.5f, viewdir.z * 0.5f + 0.5f);
+          else v = vs(fabsf(vdot(fr.nrm, viewdir)));
+        }
+      }
+    }
+    float* o = out + channels * P;
+    o[0] = v.x;
+    if (channels > 1) o[1] = v.y;
+    if (channels > 2) o[2] = v.z;
   }
 }
 

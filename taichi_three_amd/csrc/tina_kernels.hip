@@ -446,15 +446,22 @@ int tina_paint_ids(tina_ctx* c, uint32_t id_base, int n, const float color[3], c
   return TINA_OK;
 }
 
-int tina_normals(tina_ctx* c, const tina_mesh* mesh, uint32_t flags, float* d_norm, void* stream) {
+int tina_gbuffer(tina_ctx* c, const tina_mesh* mesh, uint32_t flags, int mode, int channels, const float value[3],
+                 float size, float* d_out, void* stream) {
   if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
-  if (!mesh || !d_norm) return fail(c, TINA_ERR_INVALID, "mesh or normal buffer is NULL");
-  if (flags & ~(TINA_SMOOTH | TINA_PARTICLES)) return fail(c, TINA_ERR_INVALID, "normal flags: TINA_SMOOTH, TINA_PARTICLES");
+  if (!mesh || !d_out) return fail(c, TINA_ERR_INVALID, "mesh or output buffer is NULL");
+  if (flags & ~(TINA_SMOOTH | TINA_TEX | TINA_VCOLOR | TINA_PARTICLES))
+    return fail(c, TINA_ERR_INVALID, "gbuffer flags: TINA_SMOOTH, TINA_TEX, TINA_VCOLOR, TINA_PARTICLES");
+  if (mode < 0 || mode > 9 || channels < 1 || channels > 3)
+    return fail(c, TINA_ERR_INVALID, "gbuffer: mode 0..9 and 1..3 channels");
+  if (mode == 9 && !(size > 0.0f)) return fail(c, TINA_ERR_INVALID, "gbuffer: the chessboard size must be > 0");
   if (mesh->nfaces < 0) return fail(c, TINA_ERR_INVALID, "nfaces must be >= 0");
   if (mesh->nfaces == 0) return TINA_OK;
   if (!mesh->verts) return fail(c, TINA_ERR_INVALID, "verts is NULL");
   if ((flags & (TINA_SMOOTH | TINA_PARTICLES)) && !mesh->norms)
     return fail(c, TINA_ERR_INVALID, "TINA_SMOOTH / TINA_PARTICLES need mesh->norms (normals / radii)");
+  if ((flags & TINA_TEX) && !mesh->coors) return fail(c, TINA_ERR_INVALID, "TINA_TEX without texcoords");
+  if ((flags & TINA_VCOLOR) && !mesh->colors) return fail(c, TINA_ERR_INVALID, "TINA_VCOLOR without colours");
   ShadeArgs a;
   memset(&a, 0, sizeof(a));
   a.mesh = *mesh;
@@ -464,14 +471,20 @@ int tina_normals(tina_ctx* c, const tina_mesh* mesh, uint32_t flags, float* d_no
   int rc = flush_clear(c, s);
   if (rc) return rc;
   const long npix = (long)c->W * c->H;
+  const float v0 = value ? value[0] : 0.0f, v1 = value ? value[1] : 0.0f, v2 = value ? value[2] : 0.0f;
   if (flags & TINA_PARTICLES)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_normals<true>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
-                       (const uint64_t*)c->keys, d_norm);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<true>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
+                       (const uint64_t*)c->keys, d_out, mode, channels, v0, v1, v2, size);
   else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_normals<false>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
-                       (const uint64_t*)c->keys, d_norm);
-  TINA_LAUNCH(c, "k_normals");
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<false>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
+                       (const uint64_t*)c->keys, d_out, mode, channels, v0, v1, v2, size);
+  TINA_LAUNCH(c, "k_gbuffer");
   return TINA_OK;
+}
+
+int tina_normals(tina_ctx* c, const tina_mesh* mesh, uint32_t flags, float* d_norm, void* stream) {
+  if (flags & ~(TINA_SMOOTH | TINA_PARTICLES)) return fail(c, TINA_ERR_INVALID, "normal flags: TINA_SMOOTH, TINA_PARTICLES");
+  return tina_gbuffer(c, mesh, flags, 0, 3, nullptr, 1.0f, d_norm, stream);
 }
 
 int tina_ssao(tina_ctx* c, const float* d_norm, const float* d_samples, int nsamples, const float* d_rot, int nrot,

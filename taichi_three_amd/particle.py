@@ -130,6 +130,11 @@ class ParticleRaster:
         if self._fb is None or (self.nfaces == 0 and not (extra_flags & _abi.BG)):
             return
         main = shader.main() if isinstance(shader, ShaderGroup) else shader
+        if main is None or hasattr(main, 'mode'):
+            # auxiliary shaders only (NormalShader, DepthShader, ...): no colour pass
+            if self.nfaces:
+                shader.post_shade(self, self.id_base, self.nfaces)
+            return
         if not isinstance(main, Shader):
             raise NotImplementedError(f'{type(main).__name__} is not a lowerable shader')
         eng = self.engine

@@ -7,7 +7,11 @@ ShaderGroup collects it with post shaders such as ProbeShader, which read the
 resolved id buffer after the launch.
 """
 
-__all__ = ['IShader', 'Shader', 'ShaderGroup', 'BackgroundShader', 'NormalShader']
+import numpy as np
+
+__all__ = ['IShader', 'Shader', 'ShaderGroup', 'BackgroundShader', 'NormalShader', 'ViewNormalShader',
+           'TexcoordShader', 'ColorShader', 'PositionShader', 'DepthShader', 'ConstShader', 'ViewdirShader',
+           'SimpleShader', 'ChessboardShader']
 
 
 class IShader:
@@ -41,10 +45,11 @@ class ShaderGroup(IShader):
             s.clear_buffer()
 
     def main(self):
+        """The group's Shader(img, lighting, material), or None for a group of auxiliary shaders only."""
         mains = [s for s in self.shaders if isinstance(s, Shader)]
-        if len(mains) != 1:
-            raise NotImplementedError('a ShaderGroup needs exactly one Shader(img, lighting, material)')
-        return mains[0]
+        if len(mains) > 1:
+            raise NotImplementedError('a ShaderGroup holds at most one Shader(img, lighting, material)')
+        return mains[0] if mains else None
 
     def post_shade(self, raster, id_base, nfaces):
         for s in self.shaders:
@@ -60,22 +65,95 @@ class BackgroundShader(IShader):
         self.lighting = lighting
 
 
-class NormalShader(IShader):
-    """img[P] = the shading normal (shader.py:43-46), for the pixels each object
-    won; runs after that object's colour pass (tina_normals).  Wire pixels are
-    left as they are (wires have no normal)."""
+class _BufferShader(IShader):
+    """An auxiliary shader (tina/core/shader.py:21-109): after an object's colour
+    pass (or on its own, passed to render_color) it writes one value per pixel
+    the object won into img (tina_gbuffer).  img: a Field [W, H, c] or [W, H]."""
+    mode = 0
+
+    def _value(self):
+        return None
 
     def post_shade(self, raster, id_base, nfaces):
+        import ctypes
         from . import _abi
         if nfaces == 0 or not hasattr(raster, 'mesh_struct'):
-            return
+            return                                  # wires have no fragment (IShader.blend_color)
+        t = self.img.storage
+        channels = int(t.shape[2]) if t.dim() == 3 else 1
         flags = 0
         if type(raster).__name__ == 'ParticleRaster':
-            flags = _abi.PARTICLES
-        elif getattr(raster, 'smoothing', False):
-            flags = _abi.SMOOTH
+            flags = _abi.PARTICLES | (_abi.VCOLOR if raster.coloring and raster._fb.get('colors') is not None else 0)
+        else:
+            flags |= _abi.SMOOTH if getattr(raster, 'smoothing', False) else 0
+            flags |= _abi.TEX if getattr(raster, 'texturing', False) else 0
+            flags |= _abi.VCOLOR if raster._fb is not None and raster._fb.get('colors') is not None else 0
+        v = self._value()
+        val = (ctypes.c_float * 3)(*([float(x) for x in np.broadcast_to(np.asarray(v, np.float32), 3)] if v is not None
+                                    else [0.0, 0.0, 0.0]))
         eng = raster.engine
         mesh = raster.mesh_struct()
-        import ctypes
-        _abi.check(eng._lib.tina_normals(eng._ctx, ctypes.byref(mesh), flags, _abi.ptr(self.img.storage),
-                                         _abi.stream_handle()), eng._ctx)
+        _abi.check(eng._lib.tina_gbuffer(eng._ctx, ctypes.byref(mesh), flags, self.mode, channels, ctypes.byref(val),
+                                         ctypes.c_float(getattr(self, 'size', 1)), _abi.ptr(t), _abi.stream_handle()),
+                   eng._ctx)
+
+
+class NormalShader(_BufferShader):
+    """img[P] = normal (shader.py:43-46)."""
+    mode = 0
+
+
+class ViewNormalShader(_BufferShader):
+    """img[P] = normalize(mapply_dir(W2V, normal)), its xy for a 2-channel img (shader.py:49-56)."""
+    mode = 1
+
+
+class TexcoordShader(_BufferShader):
+    """img[P] = texcoord (shader.py:59-62)."""
+    mode = 2
+
+
+class ColorShader(_BufferShader):
+    """img[P] = the Input('color') value (shader.py:65-68)."""
+    mode = 3
+
+
+class PositionShader(_BufferShader):
+    """img[P] = pos (shader.py:30-33)."""
+    mode = 4
+
+
+class DepthShader(_BufferShader):
+    """img[P] = engine.depth[P] (shader.py:36-40)."""
+    mode = 5
+
+
+class ConstShader(_BufferShader):
+    """img[P] = value (shader.py:21-27)."""
+    mode = 6
+
+    def __init__(self, img, value):
+        super().__init__(img)
+        self.value = value
+
+    def _value(self):
+        return self.value
+
+
+class ViewdirShader(_BufferShader):
+    """img[P] = viewdir * 0.5 + 0.5 (shader.py:96-101)."""
+    mode = 7
+
+
+class SimpleShader(_BufferShader):
+    """img[P] = |normal . viewdir| (shader.py:104-109)."""
+    mode = 8
+
+
+class ChessboardShader(_BufferShader):
+    """img[P] = lerp((p // size).sum() % 2, 0.4, 0.9), p = P + bias (shader.py:71-78)."""
+    mode = 9
+
+    def __init__(self, img, size=8):
+        super().__init__(img)
+        self.size = size

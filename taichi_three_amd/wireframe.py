@@ -93,6 +93,8 @@ class WireframeRaster:
         from .shader import ShaderGroup
         eng = self.engine
         main = shader.main() if isinstance(shader, ShaderGroup) else shader
+        if main is None or hasattr(main, 'mode'):
+            return  # auxiliary shaders leave wire pixels alone (IShader.blend_color)
         flags = int(extra_flags) & (_abi.BG | _abi.TONEMAP)
         if self.nfaces == 0 and not (flags & _abi.BG):
             return
