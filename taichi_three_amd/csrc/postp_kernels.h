@@ -270,10 +270,13 @@ k_accumulate(float* __restrict__ acc, const float* __restrict__ src, long n, flo
 //   3 ColorShader       color                    7 ViewdirShader   viewdir * 0.5 + 0.5
 //   4 PositionShader    pos                      8 SimpleShader    |normal . viewdir|
 //   9 ChessboardShader  lerp((p // size).sum() % 2, 0.4, 0.9), p = P + bias
-template <bool PART>
+// MODE >= 0 fixes the mode at compile time (the normal buffer SSAO reads takes
+// MODE 0); MODE = -1 reads it from `mode`.
+template <bool PART, int MODE>
 __global__ void __launch_bounds__(kBlock)
-k_gbuffer(const ShadeArgs a, const uint64_t* __restrict__ keys, float* __restrict__ out, int mode, int channels,
+k_gbuffer(const ShadeArgs a, const uint64_t* __restrict__ keys, float* __restrict__ out, int mode_arg, int channels,
           float c0, float c1, float c2, float size) {
+  const int mode = MODE >= 0 ? MODE : mode_arg;
   const long npix = (long)a.cam.W * a.cam.H;
   for (long P = (long)blockIdx.x * kBlock + threadIdx.x; P < npix; P += (long)gridDim.x * kBlock) {
     const uint64_t k = keys[P];

@@ -472,12 +472,22 @@ int tina_gbuffer(tina_ctx* c, const tina_mesh* mesh, uint32_t flags, int mode, i
   if (rc) return rc;
   const long npix = (long)c->W * c->H;
   const float v0 = value ? value[0] : 0.0f, v1 = value ? value[1] : 0.0f, v2 = value ? value[2] : 0.0f;
-  if (flags & TINA_PARTICLES)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<true>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
-                       (const uint64_t*)c->keys, d_out, mode, channels, v0, v1, v2, size);
-  else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<false>), dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, s, a,
-                       (const uint64_t*)c->keys, d_out, mode, channels, v0, v1, v2, size);
+  const dim3 grid(grid_for(npix, kBlock, 8192));
+  const bool part = (flags & TINA_PARTICLES) != 0;
+  if (mode == 0 && channels == 3) {
+    if (part)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<true, 0>), grid, dim3(kBlock), 0, s, a, (const uint64_t*)c->keys,
+                         d_out, mode, channels, v0, v1, v2, size);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<false, 0>), grid, dim3(kBlock), 0, s, a, (const uint64_t*)c->keys,
+                         d_out, mode, channels, v0, v1, v2, size);
+  } else if (part) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<true, -1>), grid, dim3(kBlock), 0, s, a, (const uint64_t*)c->keys,
+                       d_out, mode, channels, v0, v1, v2, size);
+  } else {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gbuffer<false, -1>), grid, dim3(kBlock), 0, s, a, (const uint64_t*)c->keys,
+                       d_out, mode, channels, v0, v1, v2, size);
+  }
   TINA_LAUNCH(c, "k_gbuffer");
   return TINA_OK;
 }
