@@ -221,3 +221,42 @@ def test_random_particle_and_wire_scenes(gpu, seed):
     nan = np.isnan(img)
     assert np.array_equal(nan, np.isnan(exp))
     assert np.abs(img[~nan] - exp[~nan]).max() <= COLOR_TOL_F32
+
+
+def test_boundary_rejects_bad_arguments(gpu):
+    """The particle, wire, paint, buffer and SSAO entry points reject malformed
+    calls with TINA_ERR_INVALID (raised as ValueError) before launching anything."""
+    import ctypes
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import _abi
+    eng = tina.Engine(64, 48)
+    lib, ctx = eng._lib, eng._ctx
+    pos = torch.zeros((4, 3), device='cuda')
+    rad = torch.ones(4, device='cuda')
+    img = torch.zeros((48, 64, 3), device='cuda')
+    col = (ctypes.c_float * 3)(1, 1, 1)
+    m = _abi.Mesh()
+    m.verts, m.norms, m.nfaces = _abi.ptr(pos), 0, 4
+    calls = [
+        lambda: lib.tina_raster_particles(ctx, _abi.ptr(pos), _abi.ptr(rad), 4, 0, _abi.CULL, None),   # flag
+        lambda: lib.tina_raster_particles(ctx, _abi.ptr(pos), None, 4, 0, 0, None),                    # radii
+        lambda: lib.tina_raster_particles(ctx, _abi.ptr(pos), _abi.ptr(rad), 4, 0xFFFFFFFE, 0, None),  # ids
+        lambda: lib.tina_raster_wires(ctx, _abi.ptr(pos), -1, 0, 0, None),
+        lambda: lib.tina_raster_wires(ctx, None, 2, 0, 0, None),
+        lambda: lib.tina_paint_ids(ctx, 0, 4, ctypes.byref(col), None, _abi.BG, _abi.ptr(img), None),  # bg missing
+        lambda: lib.tina_paint_ids(ctx, 0, 4, ctypes.byref(col), None, _abi.SMOOTH, _abi.ptr(img), None),
+        lambda: lib.tina_gbuffer(ctx, ctypes.byref(m), 0, 12, 3, None, ctypes.c_float(1), _abi.ptr(img), None),
+        lambda: lib.tina_gbuffer(ctx, ctypes.byref(m), 0, 9, 3, None, ctypes.c_float(0), _abi.ptr(img), None),
+        lambda: lib.tina_gbuffer(ctx, ctypes.byref(m), _abi.PARTICLES, 0, 3, None, ctypes.c_float(1), _abi.ptr(img),
+                                 None),                                                                  # no radii
+        lambda: lib.tina_ssao(ctx, _abi.ptr(img), _abi.ptr(pos), 0, _abi.ptr(rad), 2, ctypes.c_float(0.2),
+                              ctypes.c_float(0), ctypes.c_float(1), _abi.ptr(rad), None),                # nsamples
+        lambda: lib.tina_ssao_apply(_abi.ptr(img), 0, 48, _abi.ptr(rad), 4, 1, None),
+    ]
+    for k, call in enumerate(calls):
+        with pytest.raises(ValueError):
+            _abi.check(call(), ctx if k < len(calls) - 1 else None)
+    # an empty call is a no-op, not an error
+    _abi.check(lib.tina_raster_particles(ctx, None, None, 0, 0, 0, None), ctx)
+    _abi.check(lib.tina_raster_wires(ctx, None, 0, 0, 0, None), ctx)
