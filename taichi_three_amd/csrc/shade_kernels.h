@@ -81,10 +81,8 @@ __device__ __forceinline__ void shade_chunk(const ShadeArgs& a, const uint64_t* 
   }
 }
 
-// Eight waves per SIMD: the compiler would otherwise spend 106 SGPRs and hold
-// seven; at 78 SGPRs (no spills) the 1080p/2M shade pass takes 55.5 us, not 57.8.
 template <int PROG, bool IBL>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(kBlock)
 k_shade(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __restrict__ img) {
   // Read the 2 KB argument block in place (it is the first kernel argument, at
   // offset 0 of the kernarg segment): naming the by-value parameter would copy it
@@ -96,6 +94,23 @@ k_shade(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __
   const ShadeArgs& a = a_by_value;  // the host pass only needs the signature
 #endif
   shade_chunk<PROG, IBL, kShadePix>(a, keys, img, blockIdx.x);
+}
+
+// The same pass held to eight waves per SIMD, for the fixed programs without
+// IBL: they fit 64 VGPRs with no spills once the compiler is asked to (78
+// SGPRs instead of 106, which held seven waves); the 1080p/2M PBR shade takes
+// 55.5 us instead of 57.8.  The IBL, generic and particle programs would spill
+// at that budget and keep k_shade.
+template <int PROG>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_shade8(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, float* __restrict__ img) {
+  (void)a_by_value;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const ShadeArgs& a = a_by_value;
+#endif
+  shade_chunk<PROG, false, kShadePix>(a, keys, img, blockIdx.x);
 }
 
 // One scene of a batched shade: its arguments, key buffer and image.
@@ -125,7 +140,10 @@ k_bshade(const BatchShade* __restrict__ scenes, const int* __restrict__ prefix, 
 
 template <int PROG, bool IBL>
 void launch_shade(unsigned grid, hipStream_t s, const ShadeArgs& a, const uint64_t* keys, float* img) {
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<PROG, IBL>), dim3(grid), dim3(kBlock), 0, s, a, keys, img);
+  if constexpr (!IBL && PROG != PROG_GENERIC && PROG != PROG_PARTICLE)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade8<PROG>), dim3(grid), dim3(kBlock), 0, s, a, keys, img);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<PROG, IBL>), dim3(grid), dim3(kBlock), 0, s, a, keys, img);
 }
 
 inline void dispatch_shade(int prog, bool ibl, unsigned grid, hipStream_t s, const ShadeArgs& a,
