@@ -31,6 +31,15 @@ def lib():
     return _lib
 
 
+def use_native():
+    """Rebuilds the oracle -O3 -march=native for this host (liborc_native.so)
+    and routes every later call through it: the CPU baseline's build."""
+    global _lib
+    subprocess.check_call(['make', '-s', '-B', '-C', HERE, 'liborc_native.so'])
+    _lib = ctypes.CDLL(os.path.join(HERE, 'liborc_native.so'))
+    return _lib
+
+
 def _p(a):
     return ctypes.c_void_p(0 if a is None else a.ctypes.data)
 

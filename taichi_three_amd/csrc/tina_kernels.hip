@@ -220,6 +220,19 @@ int check_material(tina_ctx* c, const tina_material* m) {
   return TINA_OK;
 }
 
+// The IBL tables a shade reads: the irradiance map, every GGX level with its
+// size, and roughness brackets with a positive step (cooktorrance_ibl divides by it).
+int check_ibl(tina_ctx* c, const tina_ibl* ibl) {
+  if (!ibl || !ibl->diff || !ibl->spec[0]) return fail(c, TINA_ERR_INVALID, "the IBL tables are missing");
+  if (ibl->diff_w < 1 || ibl->diff_h < 1) return fail(c, TINA_ERR_INVALID, "IBL irradiance map has no size");
+  for (int i = 0; i < TINA_IBL_LEVELS; i++)
+    if (!ibl->spec[i] || ibl->spec_w[i] < 1 || ibl->spec_h[i] < 1)
+      return fail(c, TINA_ERR_INVALID, "IBL level missing");
+  for (int i = 0; i < TINA_IBL_LEVELS - 1; i++)
+    if (!(ibl->rough_step[i] > 0.0f)) return fail(c, TINA_ERR_INVALID, "IBL roughness step must be > 0");
+  return TINA_OK;
+}
+
 int make_shade_args(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,
                     const tina_ibl* ibl, const float bg[3], uint32_t flags, ShadeArgs& a) {
   memset(&a, 0, sizeof(a));
@@ -230,8 +243,10 @@ int make_shade_args(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat
     a.lights = *lights;
   }
   if (ibl) a.ibl = *ibl;
-  if ((flags & TINA_IBL) && (!ibl || !ibl->spec[0] || !ibl->diff))
-    return fail(c, TINA_ERR_INVALID, "TINA_IBL needs the IBL tables");
+  if (flags & TINA_IBL) {
+    const int rc = check_ibl(c, ibl);
+    if (rc) return rc;
+  }
   a.cam = make_cam(c);
   if (bg) {
     a.bg[0] = bg[0];
@@ -653,14 +668,6 @@ static int check_sun(tina_ctx* c, const tina_lights* lights) {
   return TINA_OK;
 }
 
-static int check_ibl(tina_ctx* c, const tina_ibl* ibl) {
-  if (!ibl || !ibl->diff || !ibl->spec[0]) return fail(c, TINA_ERR_INVALID, "the surrogate needs the IBL tables");
-  for (int i = 0; i < TINA_IBL_LEVELS; i++)
-    if (!ibl->spec[i] || ibl->spec_w[i] < 1 || ibl->spec_h[i] < 1)
-      return fail(c, TINA_ERR_INVALID, "IBL level missing");
-  return TINA_OK;
-}
-
 int tina_nn_samples(const tina_lights* lights, const tina_ibl* ibl, uint32_t seed, int64_t nsamples, float r_lo,
                     float r_hi, float p_metal, float specular, float* d_x, float* d_y, void* stream) {
   int rc = check_sun(nullptr, lights);
@@ -902,7 +909,8 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
   for (int i = 0; i < nscenes; i++) {
     const tina_scene_desc& d = scenes[i];
     if (d.width <= 0 || d.height <= 0 || d.width >= 32768 || d.height >= 32768 || d.nfaces < 0 || !d.mesh.verts ||
-        !d.img || d.mesh.nfaces != d.nfaces || d.mesh.id_base != 0 || (d.shade_flags & TINA_PARTICLES)) {
+        !d.img || d.mesh.nfaces != d.nfaces || d.mesh.id_base != 0 ||
+        (d.shade_flags & (TINA_PARTICLES | TINA_IBL | TINA_NOLIGHT))) {
       batch_free(b);
       return fail(nullptr, TINA_ERR_INVALID, "scene " + std::to_string(i) + ": malformed descriptor");
     }
@@ -973,7 +981,7 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     a.lights = d.lights;
     a.cam = j.cam;
     for (int k = 0; k < 3; k++) a.bg[k] = d.bg[k];
-    a.flags = d.shade_flags & ~(TINA_IBL | TINA_NOLIGHT);
+    a.flags = d.shade_flags;
     if (d.lights.n < 0 || d.lights.n > TINA_MAX_LIGHTS) {
       batch_free(b);
       return fail(nullptr, TINA_ERR_CAPACITY, "light count out of range");

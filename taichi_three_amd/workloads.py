@@ -11,6 +11,9 @@
                 PBR with per-face roughness U[0.05,1] and metallic Bernoulli(0.3),
                 per-vertex albedo U[0,1]^3; four point lights on a radius-4 sphere
                 with intensity U[0.5,2]; camera at distance 3, fov 60, aspect 16/9.  Seed 1.
+  batch_views   64 scenes of the 512x512 workload's triangle and material
+                generators, 1k-500k faces (log-uniform), resolutions from
+                {256^2, 512^2, 1024^2, 1920x1080}, cameras on the radius-3 sphere.  Seed 2.
 
 Arrays are float32 [T, 3, 3] (face, corner, xyz) like TriangleRaster.verts.
 """
@@ -113,22 +116,31 @@ BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
 
 
 def batch_views(n=64, seed=2, faces=(1000, 500_000), resolutions=BATCH_RESOLUTIONS):
-    """n independent scenes of the 1080p soup generator: face counts log-uniform
-    in `faces`, resolution drawn uniformly from `resolutions`, camera on the
-    radius-3 sphere looking at the origin (up +y)."""
+    """n independent scenes with the triangle and material generators of the
+    512x512 workload: face counts log-uniform in `faces`, equilateral triangles
+    (centres U[-1,1]^3, edge lognormal(ln 0.05, 0.5), random orientation) with
+    their face normal at every corner (smooth shading), per-vertex albedo
+    U[0,1]^3, Blinn-Phong m = 32 under one directional light (1, 2, 3) of
+    colour 0.9 and ambient 0.1; resolution drawn uniformly from `resolutions`,
+    camera on the radius-3 sphere looking at the origin (up +y)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
         T = int(round(np.exp(rng.uniform(np.log(faces[0]), np.log(faces[1])))))
         res = tuple(resolutions[int(rng.integers(len(resolutions)))])
-        w = soup_1080p(seed=int(rng.integers(2 ** 31)), res=res, nfaces=T)
+        srng = np.random.default_rng(int(rng.integers(2 ** 31)))
+        verts = triangle_soup(srng, T, np.log(0.05), 0.5)
+        norms = face_normals(verts)
+        colors = srng.uniform(0, 1, size=verts.shape).astype(np.float32)
         d = rng.normal(size=3)
         d = 3.0 * d / np.linalg.norm(d)
         view = lookat(pos=(0, 0, 0), back=tuple(d))
         proj = perspective(60, res[0] / res[1], 0.05, 500)
         W2V, V2W = camera_matrices(view, proj)
-        w.update(name='batch_view', view=view, proj=proj, W2V=W2V, V2W=V2W)
-        out.append(w)
+        out.append(Workload(name='batch_view', res=res, verts=verts, norms=norms, colors=colors, attrs={},
+                            view=view, proj=proj, W2V=W2V, V2W=V2W, smoothing=True,
+                            lights=[dict(dir=(1, 2, 3), color=(0.9, 0.9, 0.9))], ambient=(0.1, 0.1, 0.1),
+                            material='blinn'))
     return out
 
 
