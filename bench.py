@@ -311,8 +311,12 @@ def main():
     ap.add_argument("--train-steps", type=int, default=60000)
     ap.add_argument('--nfaces', type=int, default=2_000_000)
     ap.add_argument('--copy-mb', type=int, default=0,
-                    help='first run one device copy of this many MiB (a known-size kernel that '
-                         'calibrates the HBM byte counters)')
+                    help='first run three kernels of known traffic on this many MiB (a device copy, '
+                         'a sum that only reads, a gather of 64-byte rows in random order) to '
+                         'calibrate the HBM byte counters')
+    ap.add_argument('--frame-only', action='store_true',
+                    help='time the 1080p frame alone (no surrogate, batch, side passes, SSAO or CPU '
+                         'baseline): the short run the counter profiles use')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (asked for --gpus {args.gpus}): the path runs on a '
@@ -325,11 +329,17 @@ def main():
     from taichi_three_amd import workloads as wl
 
     if args.copy_mb > 0:
-        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device='cuda')
+        n = args.copy_mb << 18
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        dst.copy_(src)                       # reads and writes copy_mb MiB
+        tot = src.sum()                      # reads copy_mb MiB
+        rows = src.view(-1, 16)              # 64-byte rows
+        perm = torch.randperm(rows.shape[0], device='cuda')
         torch.cuda.synchronize()
-        del src, dst
+        dst.view(-1, 16).copy_(rows.index_select(0, perm))   # a gather of every row in random order
+        torch.cuda.synchronize()
+        del src, dst, tot, rows, perm
 
     w = wl.soup_1080p(nfaces=args.nfaces)
     W, H = w.res
@@ -394,6 +404,9 @@ def main():
     }
     del scene, mesh
     torch.cuda.empty_cache()
+    if args.frame_only:
+        print(json.dumps(out))
+        return
     if not args.no_neural:
         out['neural'] = neural_vs_analytic(max(10, args.steps // 2), args.train_steps)
     if not args.no_batch:

@@ -259,6 +259,12 @@ int make_shade_args(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat
 
 }  // namespace
 
+// The message tina_last_error(NULL) returns, for the entry points of the other
+// translation units (mesh_kernels.hip), which take no context.
+namespace tina {
+void set_global_error(const std::string& msg) { g_create_err = msg; }
+}  // namespace tina
+
 extern "C" {
 
 int tina_version(void) { return 1; }

@@ -1,12 +1,14 @@
 """ProbeShader (tina/probe.py:4-32): per-pixel face index and texcoord readback.
 
-The reference records elmid[P] = f inside render_color.  Here the shade launch
-leaves the winning global id in the packed key buffer; post_shade copies the
-object's pixels (ids in [id_base, id_base + nfaces)) into elmid as local face
-indices, which is what the reference leaves after the last object is shaded.
+The reference records elmid[P] = f and texcoord[P] inside render_color.  Here
+the shade launch leaves the winning global id in the packed key buffer;
+post_shade copies the object's pixels (ids in [id_base, id_base + nfaces)) into
+elmid as local face indices and writes their interpolated texcoords (zero
+without texturing, as the reference passes V(0, 0); tina_gbuffer's texcoord
+mode), which is what the reference leaves after the last object is shaded.
 """
 from .field import Field
-from .shader import IShader
+from .shader import IShader, gbuffer
 
 __all__ = ['ProbeShader']
 
@@ -29,3 +31,4 @@ class ProbeShader(IShader):
         local = ids.long() - id_base
         inside = (ids >= 0) & (local >= 0) & (local < nfaces)
         self.elmid.storage.copy_(torch.where(inside, local.to(torch.int32), self.elmid.storage))
+        gbuffer(raster, 2, self.texcoord.storage)

@@ -75,27 +75,31 @@ class _BufferShader(IShader):
         return None
 
     def post_shade(self, raster, id_base, nfaces):
-        import ctypes
-        from . import _abi
-        if nfaces == 0 or not hasattr(raster, 'mesh_struct'):
-            return                                  # wires have no fragment (IShader.blend_color)
-        t = self.img.storage
-        channels = int(t.shape[2]) if t.dim() == 3 else 1
-        flags = 0
-        if type(raster).__name__ == 'ParticleRaster':
-            flags = _abi.PARTICLES | (_abi.VCOLOR if raster.coloring and raster._fb.get('colors') is not None else 0)
-        else:
-            flags |= _abi.SMOOTH if getattr(raster, 'smoothing', False) else 0
-            flags |= _abi.TEX if getattr(raster, 'texturing', False) else 0
-            flags |= _abi.VCOLOR if raster._fb is not None and raster._fb.get('colors') is not None else 0
-        v = self._value()
-        val = (ctypes.c_float * 3)(*([float(x) for x in np.broadcast_to(np.asarray(v, np.float32), 3)] if v is not None
-                                    else [0.0, 0.0, 0.0]))
-        eng = raster.engine
-        mesh = raster.mesh_struct()
-        _abi.check(eng._lib.tina_gbuffer(eng._ctx, ctypes.byref(mesh), flags, self.mode, channels, ctypes.byref(val),
-                                         ctypes.c_float(getattr(self, 'size', 1)), _abi.ptr(t), _abi.stream_handle()),
-                   eng._ctx)
+        gbuffer(raster, self.mode, self.img.storage, self._value(), getattr(self, 'size', 1))
+
+
+def gbuffer(raster, mode, out, value=None, size=1):
+    """tina_gbuffer: writes mode's per-pixel value (channels = out's last dimension,
+    or 1 for [H, W]) into the device tensor `out` at every pixel the raster's
+    object won.  Wires have no fragment (IShader.blend_color only)."""
+    import ctypes
+    from . import _abi
+    if raster.nfaces == 0 or not hasattr(raster, 'mesh_struct'):
+        return
+    channels = int(out.shape[2]) if out.dim() == 3 else 1
+    flags = 0
+    if type(raster).__name__ == 'ParticleRaster':
+        flags = _abi.PARTICLES | (_abi.VCOLOR if raster.coloring and raster._fb.get('colors') is not None else 0)
+    else:
+        flags |= _abi.SMOOTH if getattr(raster, 'smoothing', False) else 0
+        flags |= _abi.TEX if getattr(raster, 'texturing', False) else 0
+        flags |= _abi.VCOLOR if raster._fb is not None and raster._fb.get('colors') is not None else 0
+    val = (ctypes.c_float * 3)(*([float(x) for x in np.broadcast_to(np.asarray(value, np.float32), 3)]
+                                if value is not None else [0.0, 0.0, 0.0]))
+    eng = raster.engine
+    mesh = raster.mesh_struct()
+    _abi.check(eng._lib.tina_gbuffer(eng._ctx, ctypes.byref(mesh), flags, mode, channels, ctypes.byref(val),
+                                     ctypes.c_float(size), _abi.ptr(out), _abi.stream_handle()), eng._ctx)
 
 
 class NormalShader(_BufferShader):

@@ -120,3 +120,63 @@ def test_auxiliary_shaders(gpu):
     r.render_color(tina.ChessboardShader(img, size=8))
     chess = np.where(((X + 0.5) // 8 + (Y + 0.5) // 8) % 2 == 1, 0.9, 0.4).astype(np.float32)
     assert np.array_equal(_np(img)[mine_t], chess[mine_t])
+
+
+def test_probe_records_ids_and_texcoords(gpu):
+    """ProbeShader (tina/probe.py:18-20) in a ShaderGroup with the colour shader:
+    elmid is the local face index and texcoord the interpolated texcoord of every
+    pixel the textured sphere won (the binary32 oracle's fragment value), -1 and
+    0 elsewhere; an untextured second object leaves texcoord zero on its pixels."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import _abi
+    from taichi_three_amd.matr import lower_material
+    W, H = 200, 150
+    print('case: probe ids and texcoords 200x150', flush=True)
+    eng = tina.Engine(W, H)
+    eng.set_camera(tina.lookat([0, 0, 0], [0.3, 0.4, 3.0]), tina.perspective(50, W / H, 0.1, 20))
+    sphere = tina.PrimitiveMesh.sphere(lons=20, lats=10)
+    fb = sphere.face_buffers()
+    v = fb['verts'].cpu().numpy()
+    rng = np.random.default_rng(8)
+    soup = np.concatenate([v[:40] + 0.01, rng.uniform(-1.2, 1.2, (60, 3, 3))]).astype(np.float32)
+    flat = tina.SimpleMesh()
+    flat.set_face_verts(soup)
+    r1 = tina.TriangleRaster(eng, smoothing=True, texturing=True)
+    r2 = tina.TriangleRaster(eng)
+    img = _img(H, W, 3)
+    probe = tina.ProbeShader((W, H))
+    lt = tina.Lighting()
+    group = tina.ShaderGroup([tina.Shader(img, lt, tina.Diffuse(color=(0.5, 0.5, 0.5))), probe])
+    eng.clear_depth()
+    probe.clear_buffer()
+    r1.set_object(sphere)
+    r1.render_occup()
+    r2.set_object(flat)
+    r2.render_occup()
+    r1.render_color(group)
+    el1 = _np(probe.elmid).copy()
+    tc1 = _np(probe.texcoord).copy()
+    r2.render_color(group)
+    torch.cuda.synchronize()
+    ids = eng.ids.to_numpy().T
+    cam = orc.camera(eng.W2V, eng.V2W, eng.bias)
+    od, oi = orc.raster(W, H, cam, v)
+    od, oi = orc.raster(W, H, cam, soup, id_base=len(v), depth=od, occup=oi)
+    assert np.array_equal(ids, oi)
+    on1 = (oi >= 0) & (oi < len(v))
+    on2 = oi >= len(v)
+    assert on1.sum() > 2000 and on2.sum() > 300
+    lo = lower_material(tina.Lamp('texcoord'), dict(attrs={}))
+    tri = orc.HostMesh(v, fb['norms'].cpu().numpy(), fb['coors'].cpu().numpy(), None)
+    exp = orc.shade(W, H, cam, tri, lo.struct, tina.Lighting().to_struct(), oi, _abi.SMOOTH | _abi.TEX)[..., :2]
+    # after the sphere's pass: its pixels hold its ids and texcoords, the rest is untouched
+    assert np.array_equal(el1, np.where(on1, oi, -1))
+    assert np.abs(tc1 - exp).max() <= 1e-6
+    assert np.all(tc1[~on1] == 0) and np.abs(tc1[on1]).max() > 0.5
+    # after the soup's pass: its pixels hold its local ids and zero texcoords
+    el = _np(probe.elmid)
+    tc = _np(probe.texcoord)
+    assert np.array_equal(el, np.where(on2, oi - len(v), np.where(on1, oi, -1)))
+    assert np.abs(tc[on1] - exp[on1]).max() <= 1e-6
+    assert np.all(tc[~on1] == 0)
