@@ -21,6 +21,7 @@
 #ifndef TINA_HIP_H
 #define TINA_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -421,6 +422,56 @@ int tina_batch_launch(tina_batch *batch, void *stream);
 int tina_batch_keys(tina_batch *batch, int scene, uint64_t **d_keys);
 /* tina_resolve over any n packed keys (a batch scene's, for instance). */
 int tina_resolve_keys(const uint64_t *d_keys, int64_t n, int32_t *d_id, int32_t *d_depth, void *stream);
+
+/* ---- mesh nodes ---------------------------------------------------------------- */
+/* The per-frame getters of the edit nodes that set_object inlines in the
+ * reference, as device passes over resident buffers.  Context-free: errors are
+ * read with tina_last_error(NULL). */
+
+/* MeshTransform.get_face_verts / get_face_norms (tina/mesh/trans.py:29-40):
+ * d_out_verts[i] = mapply_pos(trans, d_verts[i]) for nverts points, with the
+ * op order of tina/common.py:169-183 (row r: trans[r][3] * 1, then + trans[r][j]
+ * * p[j] for j = 0, 1, 2; divided by the w row); d_out_norms[i] = trans_normal @
+ * d_norms[i] (entry r: (M[r][0] n0 + M[r][1] n1) + M[r][2] n2) for nnorms normals.
+ * trans: 16 floats row-major; trans_normal: 9 floats row-major (the reference's
+ * trans_normal field).  Points and normals are [n][3] f32. */
+int tina_mesh_transform(const float trans[16], const float trans_normal[9], const float *d_verts,
+                        float *d_out_verts, int64_t nverts, const float *d_norms, float *d_out_norms,
+                        int64_t nnorms, void *stream);
+
+/* MeshFlatNormal.get_face_norms (tina/mesh/norm.py:7-14): normalized((b - a) x
+ * (c - a)) of each face of d_soup ([nfaces][3][3] f32), written once per face
+ * (corners = 1, [nfaces][3]) or to its three corners (corners = 3, [nfaces][3][3]). */
+int tina_face_normals(const float *d_soup, int64_t nfaces, int corners, float *d_out, void *stream);
+
+/* The vertex -> incidence lists MeshSmoothNormal sums over: d_list holds the
+ * incidences 3 f + c (face f, corner c) of d_faces ([nfaces][3] i32 vertex
+ * indices) grouped by vertex, each group in ascending order; vertex v's group is
+ * d_list[d_offsets[v] .. d_offsets[v + 1]).  d_offsets: nverts + 1 i32; d_list:
+ * 3 nfaces i32.  With d_temp NULL, stores the workspace size in *temp_bytes;
+ * otherwise d_temp holds *temp_bytes bytes.  Synchronises the stream (an index
+ * outside [0, nverts) is reported as TINA_ERR_INVALID).  A set-up call, once per
+ * topology. */
+int tina_vertex_incidence(const int32_t *d_faces, int64_t nfaces, int nverts, int32_t *d_offsets,
+                          int32_t *d_list, void *d_temp, size_t *temp_bytes, void *stream);
+
+/* MeshSmoothNormal._update_normal (tina/mesh/norm.py:35-47): d_out[v] = the sum
+ * of d_face_nrm[f] ([nfaces][3], tina_face_normals with corners = 1) over the
+ * incidences of v in list order -- the order the reference's loop adds them run
+ * serially, faces ascending -- then normalized.  d_out: [nverts][3] f32. */
+int tina_smooth_normals(const float *d_face_nrm, const int32_t *d_offsets, const int32_t *d_list, int nverts,
+                        float *d_out, void *stream);
+
+/* ---- diagnostics ------------------------------------------------------------ */
+/* No counterpart in the reference.  Device self-test of the shared-divisor
+ * division the kernels use wherever the reference divides (every quotient must
+ * equal the correctly rounded a / d bit for bit): runs npairs operand pairs of
+ * `mode` (0: every numerator mantissa in [1, 2) against hashed divisors in
+ * [1, 2), 2^23 pairs per divisor; 1: random bit patterns over the whole binary32
+ * range; 2: random operands with exponents in [-70, 70]) and adds to d_counts[0]
+ * the pairs that took the shared-divisor path and to d_counts[1] the pairs whose
+ * quotient differs from the compiler's division.  d_counts: 2 zeroed u64. */
+int tina_selftest_div(int mode, uint64_t npairs, uint32_t seed, uint64_t *d_counts, void *stream);
 
 #ifdef __cplusplus
 }

@@ -163,6 +163,12 @@ EXPORTS = [
     ('tina_batch_launch', _i32, [_vp, _vp]),
     ('tina_batch_keys', _i32, [_vp, _i32, ctypes.POINTER(_vp)]),
     ('tina_resolve_keys', _i32, [_vp, _i64, _vp, _vp, _vp]),
+    ('tina_selftest_div', _i32, [_i32, ctypes.c_uint64, _u32, _vp, _vp]),
+    ('tina_mesh_transform', _i32, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _vp, _vp, _i64,
+                                   _vp, _vp, _i64, _vp]),
+    ('tina_face_normals', _i32, [_vp, _i64, _i32, _vp, _vp]),
+    ('tina_vertex_incidence', _i32, [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_size_t), _vp]),
+    ('tina_smooth_normals', _i32, [_vp, _vp, _vp, _i32, _vp, _vp]),
 ]
 
 _lib = None

@@ -18,11 +18,14 @@ __device__ __forceinline__ V3 vadd(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y,
 __device__ __forceinline__ V3 vsub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ V3 vmul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
 __device__ __forceinline__ V3 vsmul(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ V3 vdivs(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ V3 vdivs(V3 a, float s) {
+  const RcpRN R = rcp_rn(s);
+  return v3(div_rn(a.x, R), div_rn(a.y, R), div_rn(a.z, R));
+}
 __device__ __forceinline__ V3 vneg(V3 a) { return v3(-a.x, -a.y, -a.z); }
 __device__ __forceinline__ float vdot(V3 a, V3 b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
 // Matrix.normalized(): v * (1 / sqrt(sum v^2))
-__device__ __forceinline__ V3 vnormalized(V3 a) { const float inv = 1.0f / sqrtf(vdot(a, a)); return vsmul(inv, a); }
+__device__ __forceinline__ V3 vnormalized(V3 a) { const float inv = fdiv_rn(1.0f, sqrtf(vdot(a, a))); return vsmul(inv, a); }
 __device__ __forceinline__ V3 vcross(V3 a, V3 b) {
   return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
@@ -36,7 +39,8 @@ __device__ __forceinline__ V3 mapply_pos(const float* __restrict__ M, V3 p, floa
   float r0, r1, r2, w;
   mapply1(M, p.x, p.y, p.z, r0, r1, r2, w);
   if (wout) *wout = w;
-  return v3(r0 / w, r1 / w, r2 / w);
+  const RcpRN R = rcp_rn(w);
+  return v3(div_rn(r0, R), div_rn(r1, R), div_rn(r2, R));
 }
 
 // bilerp (tina/common.py:140-149) over a [w][h][c] array; the indices are
@@ -66,7 +70,7 @@ __device__ __forceinline__ V3 bilerp(PTR __restrict__ img, int w, int h, int c, 
 
 // Skybox.sample for an equirect map (tina/skybox.py:23-28, 138-145)
 __device__ __forceinline__ V3 skybox_sample(gf32* __restrict__ img, int w, int h, V3 dir) {
-  const float p = atan2f(-dir.z, dir.x) / 6.2831855f;
+  const float p = fdiv_rn(atan2f(-dir.z, dir.x), 6.2831855f);
   const float cx = p - floorf(p);
   const float cy = dir.y * 0.5f + 0.5f;
   return bilerp(img, w, h, 3, 0, (float)(w - 1) * cx, (float)(h - 1) * cy);
@@ -78,14 +82,14 @@ __device__ __forceinline__ V3 skybox_sample(gf32* __restrict__ img, int w, int h
 __device__ __forceinline__ V3 phong(float m, V3 n, V3 L, V3 V) {
   const V3 rdir = vreflect(vneg(L), n);
   const float VoR = fmaxf(0.0f, vdot(V, rdir));
-  return vs((powf(VoR, m) * (m + 2.0f)) / 2.0f);
+  return vs(fdiv_rn(powf(VoR, m) * (m + 2.0f), 2.0f));
 }
 
 // BlinnPhong (extension): max(0, N.normalize(L + V))**m * (m + 8) / 8
 __device__ __forceinline__ V3 blinnphong(float m, V3 n, V3 L, V3 V) {
   const V3 h = vnormalized(vadd(L, V));
   const float NoH = fmaxf(0.0f, vdot(n, h));
-  return vs((powf(NoH, m) * (m + 8.0f)) / 8.0f);
+  return vs(fdiv_rn(powf(NoH, m) * (m + 8.0f), 8.0f));
 }
 
 // CookTorrance.sub_brdf / brdf (material.py:322-362)
@@ -99,11 +103,11 @@ __device__ __forceinline__ V3 cooktorrance(float roughness, V3 f0, V3 n, V3 L, V
   const float VoH = fminf(ONE_M_EPS, fmaxf(EPS, vdot(half, V)));
   const float alpha2 = fmaxf(1e-6f, roughness * roughness);
   const float denom = 1.0f - (NoH * NoH) * (1.0f - alpha2);
-  const float ndf = alpha2 / (denom * denom);
-  const float k = alpha2 / 2.0f;
-  float vdf = 1.0f / (((NoV * k) + 1.0f) - k);
-  vdf = vdf * (1.0f / (((NoL * k) + 1.0f) - k));
-  vdf = vdf / ((1.0f * (1.0f - alpha2)) + ((float)(4.0 * 3.141592653589793) * alpha2));
+  const float ndf = fdiv_rn(alpha2, denom * denom);
+  const float k = fdiv_rn(alpha2, 2.0f);
+  float vdf = fdiv_rn(1.0f, ((NoV * k) + 1.0f) - k);
+  vdf = vdf * fdiv_rn(1.0f, ((NoL * k) + 1.0f) - k);
+  vdf = fdiv_rn(vdf, (1.0f * (1.0f - alpha2)) + ((float)(4.0 * 3.141592653589793) * alpha2));
   // (1 - VoH)**5 as ((x * x) * (x * x)) * x: the integer power restated by
   // squaring, as the binary32 oracle does
   const float x1 = 1.0f - VoH, x2 = x1 * x1;
@@ -124,7 +128,7 @@ __device__ __forceinline__ V3 cooktorrance_ibl(const tina_ibl& ibl, float roughn
     if (lo <= roughness && roughness <= hi) {
       const V3 w1 = skybox_sample(gptr(ibl.spec[id]), ibl.spec_w[id], ibl.spec_h[id], rdir);
       const V3 w2 = skybox_sample(gptr(ibl.spec[id + 1]), ibl.spec_w[id + 1], ibl.spec_h[id + 1], rdir);
-      const float fac2 = (hi - roughness) / st;
+      const float fac2 = fdiv_rn(hi - roughness, st);
       wei = vadd(vsmul(1.0f - fac2, w2), vsmul(fac2, w1));  // lerp(fac2, wei2, wei1)
     }
   }
@@ -263,7 +267,7 @@ __device__ __forceinline__ V3 lighting(const tina_material& m, const Slots& pv, 
 
 // aces_tonemap (tina/advans.py:31-34)
 __device__ __forceinline__ float aces1(float c) {
-  return (c * ((2.51f * c) + 0.03f)) / ((c * ((2.43f * c) + 0.59f)) + 0.14f);
+  return fdiv_rn(c * ((2.51f * c) + 0.03f), (c * ((2.43f * c) + 0.59f)) + 0.14f);
 }
 __device__ __forceinline__ V3 aces(V3 c) { return v3(aces1(c.x), aces1(c.y), aces1(c.z)); }
 
@@ -340,16 +344,18 @@ __device__ __forceinline__ void fragment(const ShadeArgs& a, uint32_t f, int px,
   const float bx = (Bv.x * 0.5f + 0.5f) * resx, by = (Bv.y * 0.5f + 0.5f) * resy;
   const float cx = (Cv.x * 0.5f + 0.5f) * resx, cy = (Cv.y * 0.5f + 0.5f) * resy;
   const float nn = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
-  const float bcnx = (bx - cx) / nn, bcny = (by - cy) / nn;
-  const float canx = (cx - ax) / nn, cany = (cy - ay) / nn;
-  const float ws0 = 1.0f / wA, ws1 = 1.0f / wB, ws2 = 1.0f / wC;
+  const RcpRN Rn = rcp_rn(nn);
+  const float bcnx = div_rn(bx - cx, Rn), bcny = div_rn(by - cy, Rn);
+  const float canx = div_rn(cx - ax, Rn), cany = div_rn(cy - ay, Rn);
+  const float ws0 = fdiv_rn(1.0f, wA), ws1 = fdiv_rn(1.0f, wB), ws2 = fdiv_rn(1.0f, wC);
   const float p_x = (float)px + a.cam.bias[0], p_y = (float)py + a.cam.bias[1];
   const float w_bc = (p_x - bx) * bcny - (p_y - by) * bcnx;
   const float w_ca = (p_x - cx) * cany - (p_y - cy) * canx;
   const float t = (1.0f - w_bc) - w_ca;
   float wx = w_bc * ws0, wy = w_ca * ws1, wz = t * ws2;
   const float s = (wx + wy) + wz;
-  wx = wx / s; wy = wy / s; wz = wz / s;
+  const RcpRN Rs = rcp_rn(s);
+  wx = div_rn(wx, Rs); wy = div_rn(wy, Rs); wz = div_rn(wz, Rs);
   fr.w0 = wx; fr.w1 = wy; fr.w2 = wz;
 
   fr.pos = vadd(vadd(vsmul(wx, A), vsmul(wy, B)), vsmul(wz, C));
@@ -427,7 +433,7 @@ __device__ __forceinline__ bool particle_setup(const Cam& cam, gf32* pos, gf32* 
 // Pl = from_viewspace(from_viewport(P + bias), z) (particle.py:106-108, 134-136)
 __device__ __forceinline__ V3 particle_plane_point(const Cam& cam, int px, int py, float vz) {
   const float p_x = (float)px + cam.bias[0], p_y = (float)py + cam.bias[1];
-  const V3 Pv = v3(((p_x / (float)cam.W) * 2.0f) - 1.0f, ((p_y / (float)cam.H) * 2.0f) - 1.0f, vz);
+  const V3 Pv = v3((fdiv_rn(p_x, (float)cam.W) * 2.0f) - 1.0f, (fdiv_rn(p_y, (float)cam.H) * 2.0f) - 1.0f, vz);
   return mapply_pos(cam.V2W, Pv);
 }
 
@@ -459,7 +465,7 @@ __device__ __forceinline__ void particle_fragment(const ShadeArgs& a, uint32_t f
 // calc_viewdir (shader.py:81-93): -normalize(V2W(uv, +1) - V2W(uv, -1))
 __device__ __forceinline__ V3 view_ray(const Cam& cam, int px, int py) {
   const float p_x = (float)px + cam.bias[0], p_y = (float)py + cam.bias[1];
-  const float ux = ((p_x / (float)cam.W) * 2.0f) - 1.0f, uy = ((p_y / (float)cam.H) * 2.0f) - 1.0f;
+  const float ux = (fdiv_rn(p_x, (float)cam.W) * 2.0f) - 1.0f, uy = (fdiv_rn(p_y, (float)cam.H) * 2.0f) - 1.0f;
   const V3 ro = mapply_pos(cam.V2W, v3(ux, uy, -1.0f));
   const V3 ro1 = mapply_pos(cam.V2W, v3(ux, uy, 1.0f));
   return vnormalized(vsub(ro1, ro));

@@ -39,6 +39,49 @@ __device__ __forceinline__ int32_t key_depth(uint64_t k) {
 typedef const __attribute__((address_space(1))) float gf32;
 __device__ __forceinline__ gf32* gptr(uint64_t p) { return (gf32*)p; }
 
+// ---- correctly rounded binary32 division with a shared divisor ----------------
+//
+// The arithmetic core of the compiler's IEEE f32 division: v_rcp_f32, one
+// Newton step on the reciprocal, the quotient, and two fused corrections (the
+// sequence between v_div_scale and v_div_fixup).  The scaling and the special-
+// case fix-up around it change nothing while |a| and |d| lie in [2^-60, 2^60]:
+// the quotient stays in [2^-120, 2^120] and every remainder above 2^-84, so no
+// intermediate is subnormal or overflows.  Outside that range (zero, infinities
+// and NaN included) the compiler's division runs.  The reciprocal step depends
+// on d alone, so the quotients by one divisor share it: a quotient then costs
+// one multiply and four FMAs.  A device self-test (tina_selftest_div) compares
+// it bit for bit with the compiler's division.
+struct RcpRN {
+  float d, r;
+  bool ok;
+};
+
+__device__ __forceinline__ RcpRN rcp_rn(float d) {
+  RcpRN R;
+  R.d = d;
+  const float ad = fabsf(d);
+  R.ok = (ad >= 0x1p-60f) & (ad <= 0x1p60f);
+  const float r0 = __builtin_amdgcn_rcpf(d);
+  const float e = __builtin_fmaf(-d, r0, 1.0f);
+  R.r = __builtin_fmaf(e, r0, r0);
+  return R;
+}
+
+// RN(a / R.d), bit-identical to the compiler's a / d.
+__device__ __forceinline__ float div_rn(float a, const RcpRN& R) {
+  const float aa = fabsf(a);
+  if (R.ok & (aa >= 0x1p-60f) & (aa <= 0x1p60f)) {
+    const float q = a * R.r;
+    const float f2 = __builtin_fmaf(-R.d, q, a);
+    const float q3 = __builtin_fmaf(f2, R.r, q);
+    const float f4 = __builtin_fmaf(-R.d, q3, a);
+    return __builtin_fmaf(f4, R.r, q3);
+  }
+  return a / R.d;
+}
+
+__device__ __forceinline__ float fdiv_rn(float a, float d) { return div_rn(a, rcp_rn(d)); }
+
 // Camera passed by value as a kernel argument (lives in the kernarg segment).
 struct Cam {
   float W2V[16];
@@ -74,9 +117,10 @@ __device__ __forceinline__ bool setup_face(const Cam& cam, const float* v, uint3
   mapply1(cam.W2V, v[0], v[1], v[2], Ar0, Ar1, Ar2, Aw);
   mapply1(cam.W2V, v[3], v[4], v[5], Br0, Br1, Br2, Bw);
   mapply1(cam.W2V, v[6], v[7], v[8], Cr0, Cr1, Cr2, Cw);
-  const float Avx = Ar0 / Aw, Avy = Ar1 / Aw, Avz = Ar2 / Aw;
-  const float Bvx = Br0 / Bw, Bvy = Br1 / Bw, Bvz = Br2 / Bw;
-  const float Cvx = Cr0 / Cw, Cvy = Cr1 / Cw, Cvz = Cr2 / Cw;
+  const RcpRN RA = rcp_rn(Aw), RB = rcp_rn(Bw), RC = rcp_rn(Cw);
+  const float Avx = div_rn(Ar0, RA), Avy = div_rn(Ar1, RA), Avz = div_rn(Ar2, RA);
+  const float Bvx = div_rn(Br0, RB), Bvy = div_rn(Br1, RB), Bvz = div_rn(Br2, RB);
+  const float Cvx = div_rn(Cr0, RC), Cvy = div_rn(Cr1, RC), Cvz = div_rn(Cr2, RC);
 
   const float facing = (Bvx - Avx) * (Cvy - Avy) - (Bvy - Avy) * (Cvx - Avx);  // :95
   if ((flags & TINA_CULL) && facing <= 0.0f) return false;                     // :96-98
@@ -104,12 +148,13 @@ __device__ __forceinline__ bool setup_face(const Cam& cam, const float* v, uint3
   s.topy = to_y < cam.H - 1 ? to_y : cam.H - 1;
 
   const float n = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);               // :110-112
-  s.bcnx = (bx - cx) / n;
-  s.bcny = (by - cy) / n;
-  s.canx = (cx - ax) / n;
-  s.cany = (cy - ay) / n;
+  const RcpRN Rn = rcp_rn(n);
+  s.bcnx = div_rn(bx - cx, Rn);
+  s.bcny = div_rn(by - cy, Rn);
+  s.canx = div_rn(cx - ax, Rn);
+  s.cany = div_rn(cy - ay, Rn);
   s.bx = bx; s.by = by; s.cx = cx; s.cy = cy;
-  s.ws0 = 1.0f / Aw; s.ws1 = 1.0f / Bw; s.ws2 = 1.0f / Cw;                       // :113
+  s.ws0 = div_rn(1.0f, RA); s.ws1 = div_rn(1.0f, RB); s.ws2 = div_rn(1.0f, RC);  // :113
   s.Az = Avz; s.Bz = Bvz; s.Cz = Cvz;
   return true;
 }
@@ -122,7 +167,8 @@ __device__ __forceinline__ bool pixel_weights(const FaceSetup& s, float posx, fl
   const float t = (1.0f - w_bc) - w_ca;
   const float x = w_bc * s.ws0, y = w_ca * s.ws1, z = t * s.ws2;
   const float sum = (x + y) + z;
-  w0 = x / sum; w1 = y / sum; w2 = z / sum;
+  const RcpRN Rs = rcp_rn(sum);
+  w0 = div_rn(x, Rs); w1 = div_rn(y, Rs); w2 = div_rn(z, Rs);
   return (w0 >= 0.0f) & (w1 >= 0.0f) & (w2 >= 0.0f);
 }
 
@@ -155,7 +201,8 @@ __device__ __forceinline__ bool pixel_cover_fast(const FaceSetup& s, float posx,
     const bool cz_ = (z == 0.0f) | ((z > 0.0f) == pos);
     if (!(cx_ & cy_ & cz_)) return false;
   }
-  w0 = x / sum; w1 = y / sum; w2 = z / sum;
+  const RcpRN Rs = rcp_rn(sum);
+  w0 = div_rn(x, Rs); w1 = div_rn(y, Rs); w2 = div_rn(z, Rs);
   return (w0 >= 0.0f) & (w1 >= 0.0f) & (w2 >= 0.0f);
 }
 

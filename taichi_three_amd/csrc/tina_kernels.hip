@@ -265,6 +265,50 @@ namespace tina {
 void set_global_error(const std::string& msg) { g_create_err = msg; }
 }  // namespace tina
 
+// ---- device self-test of the shared-divisor division --------------------------
+//
+// Pair i of mode 0: numerator mantissa i mod 2^23 in [1, 2) against the hashed
+// divisor number i >> 23 in [1, 2) (every numerator against each divisor, so the
+// quotients fall at every distance from the rounding midpoints); mode 1: both
+// operands random bit patterns over the whole binary32 range, specials
+// included; mode 2: random mantissas with exponents in [-70, 70].  counts[0]
+// += pairs that took the shared-divisor path, counts[1] += pairs whose result
+// differs in any bit from the compiler's a / d (NaN matches NaN).
+__global__ void __launch_bounds__(kBlock)
+k_selftest_div(int mode, uint64_t npairs, uint32_t seed, unsigned long long* __restrict__ counts) {
+  auto h = [](uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+  };
+  unsigned long long fast = 0, bad = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < npairs; i += stride) {
+    uint32_t ab, db;
+    if (mode == 0) {
+      ab = 0x3f800000u | (uint32_t)(i & 0x7fffffu);
+      db = 0x3f800000u | (h(seed ^ h((uint32_t)(i >> 23) + 0x9e3779b9u)) & 0x7fffffu);
+    } else {
+      const uint32_t k = h(seed + (uint32_t)i * 2654435761u) ^ (uint32_t)(i >> 32);
+      ab = h(k ^ 0x85ebca6bu);
+      db = h(k ^ 0xc2b2ae35u);
+      if (mode == 2) {
+        ab = (ab & 0x807fffffu) | ((uint32_t)(127 - 70 + (int)(h(ab) % 141u)) << 23);
+        db = (db & 0x807fffffu) | ((uint32_t)(127 - 70 + (int)(h(db) % 141u)) << 23);
+      }
+    }
+    const float a = __uint_as_float(ab), d = __uint_as_float(db);
+    const RcpRN R = rcp_rn(d);
+    const float aa = fabsf(a);
+    fast += (R.ok & (aa >= 0x1p-60f) & (aa <= 0x1p60f)) ? 1u : 0u;
+    const float got = div_rn(a, R);
+    const float want = a / d;
+    const bool same = (__float_as_uint(got) == __float_as_uint(want)) || (got != got && want != want);
+    bad += same ? 0u : 1u;
+  }
+  atomicAdd(&counts[0], fast);
+  atomicAdd(&counts[1], bad);
+}
+
 extern "C" {
 
 int tina_version(void) { return 1; }
@@ -1117,6 +1161,14 @@ int tina_resolve(tina_ctx* c, int32_t* d_id, int32_t* d_depth, void* stream) {
                      d_id, d_depth, n);
   TINA_LAUNCH(c, "k_resolve");
   return TINA_OK;
+}
+
+int tina_selftest_div(int mode, uint64_t npairs, uint32_t seed, uint64_t* d_counts, void* stream) {
+  if (!d_counts || mode < 0 || mode > 2) return fail(nullptr, TINA_ERR_INVALID, "tina_selftest_div: bad mode or NULL");
+  if (npairs == 0) return TINA_OK;
+  hipLaunchKernelGGL(k_selftest_div, dim3(4096), dim3(kBlock), 0, (hipStream_t)stream, mode, npairs, seed,
+                     (unsigned long long*)d_counts);
+  return hip_check(nullptr, hipGetLastError(), "k_selftest_div");
 }
 
 }  // extern "C"
