@@ -188,3 +188,34 @@ def bake_ggx(env, ow, oh, roughness, nsamples, nthreads=8):
     lib().orc_bake_ggx(_p(env), env.shape[0], env.shape[1], _p(out), ow, oh, ctypes.c_float(roughness),
                        nsamples, nthreads)
     return out
+
+
+def mesh_transform(trans, trans_normal, verts, norms=None):
+    """MeshTransform on points [..., 3] (and normals [..., 3]): (verts', norms')."""
+    t = np.ascontiguousarray(trans, dtype=np.float32).reshape(16)
+    tn = np.ascontiguousarray(trans_normal, dtype=np.float32).reshape(9)
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    ov = np.empty_like(v)
+    n = None if norms is None else np.ascontiguousarray(norms, dtype=np.float32)
+    on = None if n is None else np.empty_like(n)
+    lib().orc_mesh_transform(_p(t), _p(tn), _p(v), _p(ov), ctypes.c_long(v.size // 3), _p(n), _p(on),
+                             ctypes.c_long(0 if n is None else n.size // 3))
+    return ov, on
+
+
+def face_normals(soup):
+    """normalized((b - a) x (c - a)) of every face of soup [nf, 3, 3]: [nf, 3]."""
+    v = np.ascontiguousarray(soup, dtype=np.float32)
+    out = np.empty((len(v), 3), dtype=np.float32)
+    lib().orc_face_normals(_p(v), ctypes.c_long(len(v)), _p(out))
+    return out
+
+
+def smooth_normals(soup, faces, nverts):
+    """Vertex normals of MeshSmoothNormal: faces [nf, 3] vertex indices, positions
+    from the soup [nf, 3, 3] (faces visited in ascending order): [nverts, 3]."""
+    v = np.ascontiguousarray(soup, dtype=np.float32)
+    f = np.ascontiguousarray(faces, dtype=np.int32)
+    out = np.empty((nverts, 3), dtype=np.float32)
+    lib().orc_smooth_normals(_p(v), _p(f), ctypes.c_long(len(f)), int(nverts), _p(out))
+    return out

@@ -460,3 +460,62 @@ void orc_bake_ggx(const float *env, int ew, int eh, float *out, int ow, int oh,
             o[0] = (float)(res.x / dem); o[1] = (float)(res.y / dem); o[2] = (float)(res.z / dem);
         }
 }
+
+/* ---------------------------------------------------------------- mesh nodes */
+
+/* mapply_pos (tina/common.py:169-183): res = mat[i][3] * 1, then res += mat[i][j] *
+ * pos[j] for j = 0..2; w likewise from row 3; res / w.  Normals: trans_normal @ n,
+ * each entry ((M[i][0] n0 + M[i][1] n1) + M[i][2] n2) (Matrix @ Vector). */
+void orc_mesh_transform(const float trans[16], const float trans_normal[9], const float *verts, float *out_verts,
+                        long nverts, const float *norms, float *out_norms, long nnorms)
+{
+    for (long i = 0; i < nverts; i++) {
+        const float *p = verts + 3 * i;
+        float r[3], w;
+        mapply1(trans, p, r, &w);
+        for (int k = 0; k < 3; k++) out_verts[3 * i + k] = r[k] / w;
+    }
+    for (long i = 0; i < nnorms; i++) {
+        const float *n = norms + 3 * i;
+        for (int k = 0; k < 3; k++) {
+            const float *m = trans_normal + 3 * k;
+            out_norms[3 * i + k] = ((m[0] * n[0]) + (m[1] * n[1])) + (m[2] * n[2]);
+        }
+    }
+}
+
+/* (b - a).cross(c - a).normalized(): Taichi's cross product, then v * (1 / |v|)
+ * with |v|^2 summed x, y, z in order. */
+static void face_normal_f32(const float *v, float n[3])
+{
+    const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+    const float e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+    const float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const float inv = 1.0f / sqrtf(((c[0] * c[0]) + (c[1] * c[1])) + (c[2] * c[2]));
+    for (int k = 0; k < 3; k++) n[k] = inv * c[k];
+}
+
+void orc_face_normals(const float *soup, long nfaces, float *out)
+{
+    for (long f = 0; f < nfaces; f++) face_normal_f32(soup + 9 * f, out + 3 * f);
+}
+
+/* norm[v] = 0; for n in range(nfaces): norm[i] += nrm; norm[j] += nrm;
+ * norm[k] += nrm; then norm[v] = norm[v].normalized()  (norm.py:37-47). */
+void orc_smooth_normals(const float *soup, const int32_t *faces, long nfaces, int nverts, float *out)
+{
+    for (long v = 0; v < 3L * nverts; v++) out[v] = 0.0f;
+    for (long f = 0; f < nfaces; f++) {
+        float n[3];
+        face_normal_f32(soup + 9 * f, n);
+        for (int c = 0; c < 3; c++) {
+            float *o = out + 3L * faces[3 * f + c];
+            for (int k = 0; k < 3; k++) o[k] = o[k] + n[k];
+        }
+    }
+    for (long v = 0; v < nverts; v++) {
+        float *o = out + 3 * v;
+        const float inv = 1.0f / sqrtf(((o[0] * o[0]) + (o[1] * o[1])) + (o[2] * o[2]));
+        for (int k = 0; k < 3; k++) o[k] = inv * o[k];
+    }
+}

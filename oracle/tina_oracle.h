@@ -92,6 +92,19 @@ void orc_bake_lambert(const float *env, int ew, int eh, float *out, int ow, int 
 void orc_bake_ggx(const float *env, int ew, int eh, float *out, int ow, int oh,
                   float roughness, int nsamples, int nthreads);
 
+/* Mesh nodes upstream of set_object (binary32, serial):
+ *   orc_mesh_transform  MeshTransform.get_face_verts / get_face_norms
+ *                       (tina/mesh/trans.py:29-40): mapply_pos (tina/common.py:169-183)
+ *                       of every position, trans_normal @ n of every normal
+ *   orc_face_normals    normalized((b - a) x (c - a)) per face (tina/mesh/norm.py:7-14)
+ *   orc_smooth_normals  MeshSmoothNormal._update_normal (norm.py:35-47) with the
+ *                       faces visited in ascending order; faces [nf][3] vertex
+ *                       indices, positions taken from the face soup [nf][3][3] */
+void orc_mesh_transform(const float trans[16], const float trans_normal[9], const float *verts, float *out_verts,
+                        long nverts, const float *norms, float *out_norms, long nnorms);
+void orc_face_normals(const float *soup, long nfaces, float *out);
+void orc_smooth_normals(const float *soup, const int32_t *faces, long nfaces, int nverts, float *out);
+
 #ifdef __cplusplus
 }
 #endif

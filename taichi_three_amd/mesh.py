@@ -10,8 +10,11 @@ Extensions beyond the reference (needed by the benchmark workloads):
   set_face_colors(c[T,3,3])   per-vertex albedo, bound to Input('color')
   set_face_attr(name, a[T] or a[T,3])   per-face arrays read by tina.FaceAttr(name)
 """
+import ctypes
+
 import numpy as np
 
+from . import _abi
 from .engine import MAX
 
 __all__ = ['SimpleMesh', 'PrimitiveMesh', 'MeshFlatNormal', 'MeshSmoothNormal', 'ConnectiveMesh',
@@ -233,7 +236,12 @@ class MeshEditBase:
 
 
 class MeshFlatNormal(MeshEditBase):
-    """Per-face normals normalized((b - a) x (c - a)) on every corner (tina/mesh/norm.py:5-14)."""
+    """Per-face normals normalized((b - a) x (c - a)) on every corner (tina/mesh/norm.py:5-14),
+    computed on the device (tina_face_normals) from the wrapped mesh's positions."""
+
+    def __init__(self, mesh):
+        super().__init__(mesh)
+        self._out = None
 
     def face_buffers(self):
         fb = dict(self.mesh.face_buffers())
@@ -241,13 +249,12 @@ class MeshFlatNormal(MeshEditBase):
         if v is None:
             return fb
         torch = _torch()
-        a, b, c = v[:, 0], v[:, 1], v[:, 2]
-        e1, e2 = b - a, c - a
-        n = torch.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1],
-                         e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
-                         e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], dim=1)
-        n = n * (1.0 / torch.sqrt((n * n).sum(dim=1, keepdim=True)))
-        fb['norms'] = n[:, None, :].expand(-1, 3, -1).contiguous()
+        nf = int(fb['nfaces'])
+        if self._out is None or self._out.shape[0] < nf or self._out.device != v.device:
+            self._out = torch.empty((max(nf, 1), 3, 3), dtype=torch.float32, device=v.device)
+        out = self._out[:nf]
+        _abi.check(_abi.load().tina_face_normals(_abi.ptr(v), nf, 3, _abi.ptr(out), _abi.stream_handle()))
+        fb['norms'] = out
         return fb
 
 
@@ -360,67 +367,129 @@ class MeshModel:
                     attrs=self.attrs, nfaces=self.maxfaces)
 
 
+def _vertex_indices(mesh):
+    """The [nf, 3] vertex indices of the indexed mesh under a chain of edit nodes
+    (get_face_vert_indices: ConnectiveMesh faces, MeshModel's vertex index column)."""
+    m = mesh
+    while not hasattr(m, 'faces') and isinstance(m, MeshEditBase):
+        m = m.mesh
+    f = getattr(m, 'faces', None)
+    if f is None or not hasattr(m, 'get_max_vert_nindex'):
+        raise NotImplementedError('MeshSmoothNormal needs an indexed mesh (ConnectiveMesh or MeshModel)')
+    return m, (f[:, :, 0] if f.dim() == 3 else f)
+
+
 class MeshSmoothNormal(MeshEditBase):
-    """Area-blind vertex normals: sum of normalized face normals, renormalised
-    (tina/mesh/norm.py:17-55).  Needs an indexed mesh (ConnectiveMesh)."""
+    """Area-blind vertex normals (tina/mesh/norm.py:17-55): per vertex the sum of
+    the normalized face normals of its faces, added in ascending face order (the
+    reference's loop run serially), then normalized.  On the device: the
+    vertex -> (face, corner) lists are built once per topology
+    (tina_vertex_incidence, a stable sort keeps each list in face order), the
+    face normals come from the wrapped mesh's current positions
+    (tina_face_normals) and each vertex sums its list in order
+    (tina_smooth_normals).  cached=True computes them once, as the reference's
+    materialize callback does (update_normal() recomputes); cached=False every
+    frame."""
 
     def __init__(self, mesh, cached=True):
         super().__init__(mesh)
-        if not hasattr(mesh, 'faces') or not hasattr(mesh, 'verts_v'):
-            raise NotImplementedError('MeshSmoothNormal needs an indexed mesh (tina.ConnectiveMesh)')
+        self._base, _ = _vertex_indices(mesh)
         self.cached = cached
         self._norm = None
         self._version = None
+        self._csr = None
+
+    def _incidence(self):
+        torch = _torch()
+        base, idx = _vertex_indices(self.mesh)
+        key = (id(base), getattr(base, 'version', 0))
+        if self._csr is not None and self._csr[0] == key:
+            return self._csr[1:]
+        lib = _abi.load()
+        faces = idx.to(torch.int32).contiguous()
+        nf, nv = int(faces.shape[0]), int(base.get_max_vert_nindex())
+        offsets = torch.empty(nv + 1, dtype=torch.int32, device=faces.device)
+        lst = torch.empty(max(3 * nf, 1), dtype=torch.int32, device=faces.device)
+        nbytes = ctypes.c_size_t(0)
+        _abi.check(lib.tina_vertex_incidence(_abi.ptr(faces), nf, nv, _abi.ptr(offsets), _abi.ptr(lst), None,
+                                             ctypes.byref(nbytes), _abi.stream_handle()))
+        temp = torch.empty(int(nbytes.value), dtype=torch.uint8, device=faces.device)
+        _abi.check(lib.tina_vertex_incidence(_abi.ptr(faces), nf, nv, _abi.ptr(offsets), _abi.ptr(lst),
+                                             _abi.ptr(temp), ctypes.byref(nbytes), _abi.stream_handle()))
+        self._csr = (key, faces.long(), offsets, lst, nv)
+        return self._csr[1:]
 
     def update_normal(self):
         torch = _torch()
-        v, f = self.mesh.verts_v, self.mesh.faces
-        a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
-        e1, e2 = b - a, c - a
-        n = torch.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1],
-                         e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
-                         e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], dim=1)
-        n = n * (1.0 / torch.sqrt((n * n).sum(dim=1, keepdim=True)))
-        acc = torch.zeros_like(v)
-        for k in range(3):
-            acc.index_add_(0, f[:, k], n)
-        self._norm = acc * (1.0 / torch.sqrt((acc * acc).sum(dim=1, keepdim=True)))
-        self._version = self.mesh.version
+        faces, offsets, lst, nv = self._incidence()
+        fb = self.mesh.face_buffers()
+        v = fb['verts']
+        nf = int(faces.shape[0])
+        lib = _abi.load()
+        fn = torch.empty((max(nf, 1), 3), dtype=torch.float32, device=v.device)
+        _abi.check(lib.tina_face_normals(_abi.ptr(v), nf, 1, _abi.ptr(fn), _abi.stream_handle()))
+        out = torch.empty((max(nv, 1), 3), dtype=torch.float32, device=v.device)
+        _abi.check(lib.tina_smooth_normals(_abi.ptr(fn), _abi.ptr(offsets), _abi.ptr(lst), nv, _abi.ptr(out),
+                                           _abi.stream_handle()))
+        self._norm = out[:nv]
+        self._version = getattr(self._base, 'version', 0)
+
+    @property
+    def norm(self):
+        """The vertex normals [nverts, 3] (MeshSmoothNormal.norm)."""
+        if self._norm is None:
+            self.update_normal()
+        return self._norm
 
     def face_buffers(self):
-        if self._norm is None or not self.cached or self._version != self.mesh.version:
+        if self._norm is None or not self.cached or self._version != getattr(self._base, 'version', 0):
             self.update_normal()
+        faces, _, _, _ = self._incidence()
         fb = dict(self.mesh.face_buffers())
-        fb['norms'] = self._norm[self.mesh.faces].contiguous()
+        fb['norms'] = self._norm[faces].contiguous()
         return fb
 
 
 class MeshTransform(MeshEditBase):
     """Applies a 4x4 transform to positions and its inverse transpose to normals
-    (tina/mesh/trans.py:5-40)."""
+    (tina/mesh/trans.py:5-40): every call of face_buffers (each set_object, as the
+    reference's get_face_verts runs inside it) transforms the wrapped mesh's faces
+    on the device (tina_mesh_transform) with the reference's mapply_pos order.
+    Both matrices are held in binary32 as the reference's fields hold them;
+    trans_normal is the upper-left 3x3 of transpose(inv(trans)) taken in float64."""
 
     def __init__(self, mesh, trans=None):
         super().__init__(mesh)
-        self.trans = np.eye(4)
-        if trans is not None:
-            self.set_transform(trans)
+        self._bufs = {}
+        self.set_transform(np.eye(4) if trans is None else trans)
 
     def set_transform(self, trans):
-        self.trans = np.asarray(trans, dtype=np.float64)
+        trans = np.asarray(trans, dtype=np.float64)
+        self.trans = trans
+        self._t32 = (ctypes.c_float * 16)(*[float(x) for x in trans.astype(np.float32).reshape(-1)])
+        tn = np.transpose(np.linalg.inv(trans))[:3, :3]
+        self._tn32 = (ctypes.c_float * 9)(*[float(x) for x in tn.astype(np.float32).reshape(-1)])
+
+    def _buf(self, name, like):
+        b = self._bufs.get(name)
+        if b is None or b.shape != like.shape or b.device != like.device:
+            b = self._bufs[name] = _torch().empty_like(like)
+        return b
 
     def face_buffers(self):
-        torch = _torch()
         fb = dict(self.mesh.face_buffers())
-        if fb['verts'] is None:
+        v, n = fb['verts'], fb['norms']
+        if v is None:
             return fb
-        M = torch.as_tensor(self.trans.astype(np.float32), device=fb['verts'].device)
-        v = fb['verts']
-        r = v @ M[:3, :3].T + M[:3, 3]
-        w = v @ M[3, :3] + M[3, 3]
-        fb['verts'] = (r / w[..., None]).contiguous()
-        if fb['norms'] is not None:
-            N = torch.as_tensor(np.linalg.inv(self.trans)[:3, :3].T.astype(np.float32), device=v.device)
-            fb['norms'] = (fb['norms'] @ N.T).contiguous()
+        ov = self._buf('verts', v)
+        on = self._buf('norms', n) if n is not None else None
+        _abi.check(_abi.load().tina_mesh_transform(
+            self._t32, self._tn32, _abi.ptr(v), _abi.ptr(ov), v.numel() // 3,
+            _abi.ptr(n) if n is not None else None, _abi.ptr(on) if on is not None else None,
+            0 if n is None else n.numel() // 3, _abi.stream_handle()))
+        fb['verts'] = ov
+        if on is not None:
+            fb['norms'] = on
         return fb
 
 
