@@ -296,6 +296,19 @@ int tina_mlp_train_step(const float *d_x, const float *d_y, int64_t first, int64
                         float *d_params, float *d_m1, float *d_m2, float *d_partial, float lr, int step,
                         float *d_grad, void *stream);
 
+/* No counterpart in the reference: tina_mlp_train_step for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The per-workgroup gradient rows live in a static
+ * device array of the library (TINA_MLP_STATIC_MAX_BATCH / 64 rows of
+ * TINA_NN_PARAMS floats, 9.6 MB per GPU, held until the process ends) instead of
+ * a caller workspace; a batch above TINA_MLP_STATIC_MAX_BATCH samples is
+ * rejected with TINA_ERR_CAPACITY.  Allocates nothing.  Every call on one GPU
+ * shares that array: calls that overlap on one device race. */
+#define TINA_MLP_STATIC_MAX_BATCH 16384
+int tina_mlp_train_step_static(const float *d_x, const float *d_y, int64_t first, int64_t nsamples, int batch,
+                               float *d_params, float *d_m1, float *d_m2, float lr, int step, float *d_grad,
+                               void *stream);
+
 /* Blooming.apply (tina/postp/blooming.py:45-68) on d_img [height][width][3]:
  * the bright pass factor * (1 - 1 / (1 + scale * max(0, c - thresh))) averaged
  * over 2x2 blocks into d_half, blurred along x into d_tmp and along y back into

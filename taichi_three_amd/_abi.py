@@ -140,6 +140,8 @@ EXPORTS = [
                                ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),
     ('tina_mlp_train_step', _i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp, ctypes.c_float, _i32, _vp,
                                    _vp]),
+    ('tina_mlp_train_step_static', _i32, [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp, ctypes.c_float, _i32, _vp,
+                                          _vp]),
     ('tina_mlp_pack', _i32, [_vp, _vp, _vp]),
     ('tina_bloom', _i32, [_vp, _i32, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _i32, _vp, _vp,
                           _vp]),

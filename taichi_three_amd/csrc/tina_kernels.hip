@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -774,6 +775,52 @@ int tina_mlp_train_step(const float* d_x, const float* d_y, int64_t first, int64
   hipLaunchKernelGGL(k_mlp_adam, dim3((kNnParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_params, d_m1,
                      d_m2, d_partial, nblocks, lr, (float)b1, (float)b2, 1e-8f, c1, c2, d_grad);
   return hip_check(nullptr, hipGetLastError(), "k_mlp_adam");
+}
+
+}  // extern "C"
+
+// ---- the training step without a workspace argument ---------------------------
+//
+// The gradient rows of k_mlp_grad live in a file-scope device array instead of a
+// caller's workspace: kStaticRows rows of kNnParams floats (9.6 MB on every GPU
+// the library runs on, held until the process ends).  In host code `scratch`
+// names a host-side stand-in; the kernels get the device address that
+// hipGetSymbolAddress returns, taken once per device on the entry point's first
+// call there.  Every row a step uses is written in full by k_mlp_grad before
+// k_mlp_adam reads it, so nothing needs clearing; calls that overlap on one GPU
+// share the array and race.
+namespace {
+constexpr int kStaticRows = 256;  // batch <= 256 * kNnBatchPerBlock = 16384 samples
+static __device__ __align__(256) float scratch[(size_t)kStaticRows * kNnParams];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+extern "C" {
+
+int tina_mlp_train_step_static(const float* d_x, const float* d_y, int64_t first, int64_t nsamples, int batch,
+                               float* d_params, float* d_m1, float* d_m2, float lr, int step, float* d_grad,
+                               void* stream) {
+  if (batch > kStaticRows * kNnBatchPerBlock)
+    return fail(nullptr, TINA_ERR_CAPACITY, "tina_mlp_train_step_static: batch above " +
+                                                std::to_string(kStaticRows * kNnBatchPerBlock) + " samples");
+  int dev = 0;
+  if (hip_check(nullptr, hipGetDevice(&dev), "hipGetDevice")) return TINA_ERR_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, TINA_ERR_CAPACITY, "device index above 63");
+  float* rows = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      if (hip_check(nullptr, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+        return TINA_ERR_HIP;
+      g_scratch_ptr[dev] = (float*)p;
+    }
+    rows = g_scratch_ptr[dev];
+  }
+  return tina_mlp_train_step(d_x, d_y, first, nsamples, batch, d_params, d_m1, d_m2, rows, lr, step, d_grad,
+                             stream);
 }
 
 int tina_mlp_pack(const float* d_params, void* d_packed, void* stream) {
