@@ -11,13 +11,16 @@
 //            tiles go to the giant queue (k_giant, global atomics) instead.
 //   k_tile   one workgroup per screen tile.  The tile's chunks from all binning
 //            workgroups form one virtual list (prefix sum of their counts in LDS).
-//            The tile's keys are loaded into LDS once; each wave then pulls 64 faces
+//            The tile's keys are loaded into LDS once (or start empty when the
+//            frame's clear is deferred to this call); each wave then pulls 64 faces
 //            at a time from the list, reads their setup records, clips their bboxes
 //            to the tile and rasterises them with the wave-cooperative fragment
 //            expansion (the bbox pixels of the wave's faces laid end to end, one
 //            pixel per lane per step), reducing every covered pixel with an LDS
 //            64-bit atomicMin.  Waves run independently (no workgroup barrier in
 //            the loop).  The keys go back to HBM once, coalesced.
+//   k_giant  after k_tile: the faces whose bbox spans more than max_tiles tiles,
+//            with global 64-bit atomics.
 //
 // No global atomic touches the key buffer on this path: the 64-bit atomic rate
 // to HBM-resident lines (about 27 G/s measured on MI355X) would bound the pass.
@@ -51,8 +54,10 @@ inline int bin_faces_for(long ntiles) {
   return 0;
 }
 
+// cursor: entries reserved by the binning workgroups; giant: faces queued for
+// k_giant; done: k_giant workgroups finished (the last one resets all three).
 struct BinCounters {
-  uint32_t cursor, giant, pad0, pad1;
+  uint32_t cursor, giant, done, pad;
 };
 
 // Setup record of a binned face (64 B, one cache line): the state the coverage
@@ -83,7 +88,9 @@ struct BinJob {
   uint32_t id_base, flags;
   int tiles_x, ntiles, ngroups;
   int bin_faces, max_tiles;
-  int clear_keys;  // k_bin first resets the W x H key buffer (a clear deferred to this call)
+  int load_keys;   // 1: k_tile merges into the keys as they stand; 0: the keys are
+                   // cleared (a clear deferred to this call), so k_tile starts every
+                   // tile from empty keys and writes all of them, empty tiles included
 };
 
 // The tiles a face's bbox touches: x0 (12 bits), y0 (12 bits), x span - 1 (4 bits),
@@ -278,8 +285,8 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
 }
 
 // Faces too large to bin: every lane of the grid takes a slice of each one and
-// reduces into the key buffer with global atomics.  Runs between k_bin and
-// k_tile (k_tile loads the keys after it and resets the queue).
+// reduces into the key buffer with global atomics.  Runs after k_tile has stored
+// its tiles (a minimum does not depend on the order it is taken in).
 __device__ __forceinline__ void giant_body(const BinJob& j, uint32_t ng) {
   const long stride = (long)gridDim.x * kBlock;
   for (uint32_t r = 0; r < ng; r++) {
@@ -326,15 +333,22 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   const uint32_t total = block_exclusive_scan<kBlock>(
       ngroups, [&](int c) { return chunk_cnt[(size_t)c * ntiles + t]; },
       [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);
-  if (total == 0) return;
+  const int tw = min(kTile, cam.W - ox), th = min(kTile, cam.H - oy);
+  if (total == 0) {
+    if (!j.load_keys)  // the deferred clear of a tile no face touches
+      for (int i = tid; i < kTile * kTile; i += kBlock) {
+        const int lx = i & (kTile - 1), ly = i >> kTileLog;
+        if (lx < tw && ly < th) keys[(size_t)(oy + ly) * cam.W + ox + lx] = empty_key();
+      }
+    return;
+  }
   if (tid == 0) {
     pre[ngroups] = total;
     s_next = 0;
   }
-  const int tw = min(kTile, cam.W - ox), th = min(kTile, cam.H - oy);
   for (int i = tid; i < kTile * kTile; i += kBlock) {
     const int lx = i & (kTile - 1), ly = i >> kTileLog;
-    tk[i] = (lx < tw && ly < th) ? keys[(size_t)(oy + ly) * cam.W + ox + lx] : empty_key();
+    tk[i] = (j.load_keys && lx < tw && ly < th) ? keys[(size_t)(oy + ly) * cam.W + ox + lx] : empty_key();
   }
   __syncthreads();
 
@@ -484,19 +498,31 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
 
 __global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
-  if (j.clear_keys) {  // nothing in this kernel reads the keys; k_giant and k_tile run after it
-    const long n = (long)j.cam.W * j.cam.H, stride = (long)gridDim.x * kBinBlock;
-    const uint64_t e = empty_key();
-    ulonglong2* k2 = reinterpret_cast<ulonglong2*>(j.keys);
-    for (long i = (long)blockIdx.x * kBinBlock + threadIdx.x; i < (n >> 1); i += stride) k2[i] = make_ulonglong2(e, e);
-    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) j.keys[n - 1] = e;
-  }
   bin_group(j, blockIdx.x);
+}
+
+// The last workgroup of the giant pass to finish empties the counters for the
+// next raster call; every workgroup has read the giant count before it takes its
+// ticket.
+__device__ __forceinline__ void giant_ticket(BinCounters* ctr) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = atomicAdd(&ctr->done, 1u);
+    if (t == gridDim.x - 1) {
+      __atomic_store_n(&ctr->cursor, 0u, __ATOMIC_RELAXED);
+      __atomic_store_n(&ctr->giant, 0u, __ATOMIC_RELAXED);
+      __atomic_store_n(&ctr->done, 0u, __ATOMIC_RELAXED);
+    }
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_giant(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
-  giant_body(j, j.ctr->giant);
+  __shared__ uint32_t s_ng;
+  if (threadIdx.x == 0) s_ng = __atomic_load_n(&j.ctr->giant, __ATOMIC_RELAXED);
+  __syncthreads();
+  giant_body(j, s_ng);
+  giant_ticket(j.ctr);
 }
 
 // Workgroup b of n -> tile.  The dispatcher deals workgroups round-robin over
@@ -514,11 +540,6 @@ __device__ __forceinline__ long strip_item(long b, long n) {
 
 __global__ void __launch_bounds__(kBlock) k_tile(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
-  // every binning workgroup has finished: the next raster call starts from empty queues
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    j.ctr->cursor = 0;
-    j.ctr->giant = 0;
-  }
   tile_body(j, (int)strip_item(blockIdx.x, gridDim.x));
 }
 
@@ -548,21 +569,28 @@ k_bbin(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njob
 }
 
 // The giant counts of all scenes are read at once (a batch holds at most kBlock
-// scenes) rather than one dependent load per scene.
+// scenes) rather than one dependent load per scene.  Runs after k_btile; the
+// last workgroup to finish empties every scene's counters (scene 0's `done`
+// counts the workgroups).
 __global__ void __launch_bounds__(kBlock) k_bgiant(const BinJob* __restrict__ jobs, int njobs) {
   __shared__ uint32_t s_ng[kBlock];
-  if (threadIdx.x < njobs) s_ng[threadIdx.x] = jobs[threadIdx.x].ctr->giant;
+  if (threadIdx.x < njobs) s_ng[threadIdx.x] = __atomic_load_n(&jobs[threadIdx.x].ctr->giant, __ATOMIC_RELAXED);
   __syncthreads();
   for (int s = 0; s < njobs; s++)
     if (s_ng[s]) giant_body(jobs[s], s_ng[s]);
+  __syncthreads();
+  __shared__ uint32_t s_last;
+  if (threadIdx.x == 0) s_last = atomicAdd(&jobs[0].ctr->done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (s_last && threadIdx.x < njobs) {
+    __atomic_store_n(&jobs[threadIdx.x].ctr->cursor, 0u, __ATOMIC_RELAXED);
+    __atomic_store_n(&jobs[threadIdx.x].ctr->giant, 0u, __ATOMIC_RELAXED);
+    __atomic_store_n(&jobs[threadIdx.x].ctr->done, 0u, __ATOMIC_RELAXED);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_btile(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
-  if (blockIdx.x == 0 && threadIdx.x < njobs) {
-    jobs[threadIdx.x].ctr->cursor = 0;
-    jobs[threadIdx.x].ctr->giant = 0;
-  }
   const long total = prefix[njobs];
   for (long b = blockIdx.x; b < total; b += gridDim.x) {
     const long sweep = b - blockIdx.x, width = min((long)gridDim.x, total - sweep);

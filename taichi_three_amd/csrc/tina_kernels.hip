@@ -48,7 +48,8 @@ struct tina_ctx {
   int fb_which = 0;
   int nn_grid = 0;
   // tina_clear only records the clear; the next call that touches the keys applies it
-  // (the binned raster inside k_bin, anything else with k_clear first)
+  // (the binned raster by starting its tiles from empty keys, anything else with
+  // k_clear first)
   bool clear_pending = false;
   hipStream_t clear_stream = nullptr;
   tina_camera cam{};
@@ -435,16 +436,18 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.bin_faces = bin_faces_for(j.ntiles);
     j.ngroups = (nfaces + j.bin_faces - 1) / j.bin_faces;
     j.max_tiles = kBinMaxTiles;
-    j.clear_keys = c->clear_pending ? 1 : 0;
+    // a clear recorded by tina_clear is applied by k_tile itself: it starts every
+    // tile from empty keys and writes all of them
+    j.load_keys = c->clear_pending ? 0 : 1;
     c->clear_pending = false;
     allow_big_lds();
     hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + j.bin_faces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
+    hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), (size_t)(j.ngroups + 1) * 4, s, j);
+    TINA_LAUNCH(c, "k_tile");
     // usually no face is giant: a quarter of the queue grid dispatches in a quarter of the time
     hipLaunchKernelGGL(k_giant, dim3(kQueueGrid / 4), dim3(kBlock), 0, s, j);
     TINA_LAUNCH(c, "k_giant");
-    hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), (size_t)(j.ngroups + 1) * 4, s, j);
-    TINA_LAUNCH(c, "k_tile");
     return TINA_OK;
   }
   // resolutions or face counts beyond the binned path's LDS tables: global atomics
@@ -1070,7 +1073,7 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     j.ngroups = ngroups;
     j.bin_faces = bf;
     j.max_tiles = tune->max_tiles;
-    j.clear_keys = 0;
+    j.load_keys = 0;  // k_btile writes every tile of every scene: no clear pass
     ShadeArgs& a = sh[i].a;
     memset(&a, 0, sizeof(a));
     a.mesh = d.mesh;
@@ -1126,19 +1129,17 @@ int tina_batch_render(tina_batch* b, void* stream) {
   if (!b) return fail(nullptr, TINA_ERR_INVALID, "batch is NULL");
   hipStream_t s = (hipStream_t)stream;
   const int n = b->n;
-  hipLaunchKernelGGL(k_clear, dim3(grid_for(b->total_pix / 2 + 1, kBlock, 8192)), dim3(kBlock), 0, s, b->keys,
-                     b->total_pix);
-  int rc = hip_check(nullptr, hipGetLastError(), "k_clear");
-  if (rc) return rc;
+  int rc;
   allow_big_lds();
   hipLaunchKernelGGL(k_bbin, dim3(batch_grid(b->tune.grid_bin, b->totals[0])), dim3(kBinBlock),
                      (size_t)(b->max_ntiles + b->tune.bin_faces) * 4, s, b->d_jobs, b->d_prefix, n);
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_bbin"))) return rc;
-  hipLaunchKernelGGL(k_bgiant, dim3(kQueueGrid), dim3(kBlock), 0, s, b->d_jobs, n);
-  if ((rc = hip_check(nullptr, hipGetLastError(), "k_bgiant"))) return rc;
+  // every tile of every scene starts from empty keys and is written whole: no clear pass
   hipLaunchKernelGGL(k_btile, dim3(batch_grid(b->tune.grid_tile, b->totals[1])), dim3(kBlock),
                      (size_t)(b->max_groups + 1) * 4, s, b->d_jobs, b->d_prefix + (n + 1), n);
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_btile"))) return rc;
+  hipLaunchKernelGGL(k_bgiant, dim3(kQueueGrid), dim3(kBlock), 0, s, b->d_jobs, n);
+  if ((rc = hip_check(nullptr, hipGetLastError(), "k_bgiant"))) return rc;
   const unsigned gs = batch_grid(b->tune.grid_shade, b->totals[2]);
   const int* pre = b->d_prefix + 2 * (n + 1);
   switch (b->prog) {
