@@ -222,6 +222,25 @@ int tina_normals(tina_ctx *ctx, const tina_mesh *mesh, uint32_t flags, float *d_
 int tina_gbuffer(tina_ctx *ctx, const tina_mesh *mesh, uint32_t flags, int mode, int channels,
                  const float value[3], float size, float *d_out, void *stream);
 
+/* SSR.render (tina/postp/ssr.py:51-103) for the pixels whose winning id lies in
+ * [mesh->id_base, +mesh->nfaces): per pixel, unless its normal in d_norm
+ * ([H][W][3]) is zero (then 0), nsamples directions drawn from the material's
+ * sample() (Lambert, Phong, CookTorrance, Mix, Scale, Add, Emission;
+ * material.py) at the view position of the depth buffer, each marched nsteps
+ * steps of stepsize / (sqrt(1 - (d.v)^2) nsteps); a step within `tolerance` view
+ * steps behind the depth buffer takes bilerp(d_image) times the sample weight.
+ * Writes (rgb sum, hits) / nsamples to d_out ([H][W][4]).  d_coor: [H][W][2]
+ * texcoords (TexcoordShader) or NULL.  taa = 0 draws the reference's
+ * WangHashRNG(P % blur) stream; taa = 1 a stream hashed from the pixel and seed. */
+int tina_ssr(tina_ctx *ctx, const tina_mesh *mesh, const tina_material *mat, const float *d_norm,
+             const float *d_coor, const float *d_image, int nsamples, int nsteps, float stepsize, float tolerance,
+             int blur, int taa, uint32_t seed, float *d_out, void *stream);
+
+/* SSR.apply (ssr.py:31-45): res = the blur x blur mean of d_ssr at offset
+ * -blur / 2 (reads clamped to the screen; the pixel itself with taa = 1);
+ * d_img = d_img * (1 - res.w) + res.xyz. */
+int tina_ssr_apply(float *d_img, int width, int height, const float *d_ssr, int blur, int taa, void *stream);
+
 /* SSAO.render (tina/postp/ssao.py:61-96): ambient occlusion per pixel from the
  * depth buffer and d_norm, with nsamples hemisphere samples d_samples [n][3]
  * turned by the nrot x nrot tile d_rot [nrot][nrot][2] (cos, sin); writes

@@ -219,3 +219,29 @@ def smooth_normals(soup, faces, nverts):
     out = np.empty((nverts, 3), dtype=np.float32)
     lib().orc_smooth_normals(_p(v), _p(f), ctypes.c_long(len(f)), int(nverts), _p(out))
     return out
+
+
+def ssr(W, H, cam, mesh, material, depth, occup, norm, image, coor=None, nsamples=32, nsteps=32, stepsize=2.0,
+        tolerance=15.0, blur=4, out=None):
+    """SSR.render_at (orc_ssr) for the pixels of `mesh` (a HostMesh): returns /
+    fills out [H, W, 4] (rgb sum, hits) / nsamples; other pixels left as they are."""
+    depth = np.ascontiguousarray(depth, dtype=np.int32)
+    occup = np.ascontiguousarray(occup, dtype=np.int32)
+    norm = np.ascontiguousarray(norm, dtype=np.float32)
+    image = np.ascontiguousarray(image, dtype=np.float32)
+    coor = None if coor is None else np.ascontiguousarray(coor, dtype=np.float32)
+    if out is None:
+        out = np.zeros((H, W, 4), dtype=np.float32)
+    lib().orc_ssr(W, H, ctypes.byref(cam), ctypes.byref(mesh.struct), ctypes.byref(material), _p(depth), _p(occup),
+                  _p(norm), _p(coor), _p(image), int(nsamples), int(nsteps), ctypes.c_float(stepsize),
+                  ctypes.c_float(tolerance), int(blur), _p(out))
+    return out
+
+
+def ssr_apply(image, buf, blur=4):
+    """SSR.apply (orc_ssr_apply): image * (1 - a) + rgb of the blur x blur box; returns a new image."""
+    img = np.array(image, dtype=np.float32, order='C')
+    buf = np.ascontiguousarray(buf, dtype=np.float32)
+    H, W = img.shape[:2]
+    lib().orc_ssr_apply(W, H, _p(img), _p(buf), int(blur))
+    return img

@@ -461,6 +461,252 @@ void orc_bake_ggx(const float *env, int ew, int eh, float *out, int ow, int oh,
         }
 }
 
+/* ---------------------------------------------------------------- SSR */
+
+/* WangHashRNG (tina/random.py:17-64) */
+static uint32_t wang_noise(uint32_t v)
+{
+    v = (v ^ 61u) ^ (v >> 16);
+    v *= 9u;
+    v ^= v << 4;
+    v *= 0x27d4eb2du;
+    v ^= v >> 15;
+    return v;
+}
+static float wang_random(uint32_t *seed)
+{
+    uint32_t u = wang_noise(*seed) >> 1;
+    *seed += 1u;
+    return (float)u * 4.656612873077393e-10f;    /* u * (2 / 2**32) */
+}
+static uint32_t wang_random_int(uint32_t *seed)
+{
+    uint32_t u = wang_noise(*seed);
+    *seed += 1u;
+    return u;
+}
+
+/* spherical(h, p) (tina/advans.py:105-108); cos and sin in binary64, rounded */
+static vec_f32 spherical_f32(float h, float p)
+{
+    float a = p * 6.2831855f;
+    float c = (float)cos((double)a), sn = (float)sin((double)a);
+    float r = sqrtf(fmaxf(0.0f, 1.0f - h * h));
+    return v3_f32(r * c, r * sn, h);
+}
+
+/* tangentspace(n) @ d (advans.py:97-100): columns tan, bitan, n */
+static vec_f32 tangent_apply_f32(vec_f32 n, vec_f32 d)
+{
+    vec_f32 bitan = vnormalized_f32(vcross_f32(n, v3_f32(233.0f, 666.0f, 512.0f)));
+    vec_f32 tan = vcross_f32(bitan, n);
+    return v3_f32(((tan.x * d.x) + (bitan.x * d.y)) + (n.x * d.z), ((tan.y * d.x) + (bitan.y * d.y)) + (n.y * d.z),
+                  ((tan.z * d.x) + (bitan.z * d.y)) + (n.z * d.z));
+}
+
+/* material.sample(idir, nrm, 1, rng) of the program (material.py:25-32, 124-138,
+ * 181-184, 228-238, 365-384, 398-405, 467-477), walked recursively. */
+static void sample_node(const tina_material *m, const vec_f32 *pv, int node, const int *size, vec_f32 idir,
+                        vec_f32 nrm, uint32_t *rng, vec_f32 *odir, vec_f32 *wei)
+{
+    const tina_op *o = &m->ops[node];
+    if (o->op == TINA_OP_SCALE) {
+        sample_node(m, pv, node - 1, size, idir, nrm, rng, odir, wei);
+        *wei = vmul_f32(*wei, pv[o->a]);
+        return;
+    }
+    if (o->op == TINA_OP_MIX) {
+        vec_f32 fac = pv[o->a];
+        int scalar = m->params[o->a].dim == 1;
+        float f = scalar ? fac.x : ((fac.x + fac.y) + fac.z) / 3.0f;
+        if (f != 0.0f && f != 1.0f) {
+            float t = (f - 0.0f) / (1.0f - 0.0f);
+            t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+            float sm = (t * t) * (3.0f - 2.0f * t);
+            f = 0.12f * (1.0f - sm) + 0.88f * sm;
+        }
+        if (wang_random(rng) < f) {
+            sample_node(m, pv, node - 1, size, idir, nrm, rng, odir, wei);
+            vec_f32 k = scalar ? vs_f32(fac.x / f) : vdivs_f32(fac, f);
+            *wei = vmul_f32(*wei, k);
+        } else {
+            sample_node(m, pv, node - 1 - size[node - 1], size, idir, nrm, rng, odir, wei);
+            vec_f32 k = scalar ? vs_f32((1.0f - fac.x) / (1.0f - f)) : vdivs_f32(vsub_f32(vs_f32(1.0f), fac), 1.0f - f);
+            *wei = vmul_f32(*wei, k);
+        }
+        return;
+    }
+    if (o->op == TINA_OP_ADD) {
+        int first = (wang_random_int(rng) % 2u) == 0u;
+        sample_node(m, pv, first ? node - 1 - size[node - 1] : node - 1, size, idir, nrm, rng, odir, wei);
+        *wei = vmul_f32(*wei, vs_f32(2.0f));
+        return;
+    }
+    *odir = idir;
+    *wei = vs_f32(0.0f);
+    if (o->op == TINA_OP_LAMBERT) {
+        float u = wang_random(rng);
+        float v = wang_random(rng);
+        u = sqrtf(u);
+        *odir = vnormalized_f32(tangent_apply_f32(nrm, spherical_f32(u, v)));
+        *wei = vs_f32(1.0f);
+    } else if (o->op == TINA_OP_COOKTORRANCE) {
+        float roughness = pv[o->a].x;
+        vec_f32 f0 = pv[o->b];
+        float alpha2 = fmaxf(0.0f, roughness * roughness);
+        float u = wang_random(rng);
+        float v = wang_random(rng);
+        u = sqrtf((1.0f - u) / (1.0f - u * (1.0f - alpha2)));
+        vec_f32 rdir = reflect_f32(vneg_f32(idir), nrm);
+        vec_f32 od = tangent_apply_f32(rdir, spherical_f32(u, v));
+        /* sub_brdf(nrm, idir, od): the Fresnel and visibility terms */
+        const float EPS = 1e-10f, ONE_M_EPS = (float)(1.0 - 1e-10);
+        vec_f32 half = vnormalized_f32(vadd_f32(idir, od));
+        float NoL = fmaxf(EPS, vdot_f32(idir, nrm));
+        float NoV = fmaxf(EPS, vdot_f32(od, nrm));
+        float VoH = fminf(ONE_M_EPS, fmaxf(EPS, vdot_f32(half, od)));
+        float a2 = fmaxf(1e-6f, roughness * roughness);
+        float k = a2 / 2.0f;
+        float vdf = 1.0f / (((NoV * k) + 1.0f) - k);
+        vdf = vdf * (1.0f / (((NoL * k) + 1.0f) - k));
+        vdf = vdf / ((1.0f * (1.0f - a2)) + ((float)(4.0 * 3.141592653589793) * a2));
+        float x1 = 1.0f - VoH, x2 = x1 * x1, p5 = (x2 * x2) * x1;
+        vec_f32 fdf = vadd_f32(f0, vsmul_f32(p5, vsub_f32(vs_f32(1.0f), f0)));
+        if (vdot_f32(od, nrm) < 0.0f) {
+            od = vneg_f32(od);
+            fdf = vs_f32(0.0f);
+        }
+        *odir = od;
+        *wei = vsmul_f32(vdf, fdf);
+    } else if (o->op == TINA_OP_PHONG || o->op == TINA_OP_BLINNPHONG) {
+        float mm = pv[o->a].x;
+        float u = wang_random(rng);
+        float v = wang_random(rng);
+        u = (float)pow((double)u, (double)(1.0f / (mm + 1.0f)));
+        vec_f32 rdir = reflect_f32(vneg_f32(idir), nrm);
+        vec_f32 od = tangent_apply_f32(rdir, spherical_f32(u, v));
+        float w = 1.0f;
+        if (vdot_f32(od, nrm) < 0.0f) {
+            od = vneg_f32(od);
+            w = 0.0f;
+        }
+        *odir = od;
+        *wei = vs_f32(w);
+    }
+}
+
+static vec_f32 screen_px(const float *img, int W, int x, int y)
+{
+    const float *p = img + ((long)y * W + x) * 3;
+    return v3_f32(p[0], p[1], p[2]);
+}
+
+/* bilerp (tina/common.py:140-149) over the [H][W][3] image, texels clamped */
+static vec_f32 screen_bilerp(const float *img, int W, int H, float px, float py)
+{
+    int ix = f2i_sat(floorf(px)), iy = f2i_sat(floorf(py));
+    float x0 = px - (float)ix, x1 = py - (float)iy;
+    float y0 = 1.0f - x0, y1 = 1.0f - x1;
+    int xa = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix), xb = ix + 1 < 0 ? 0 : (ix + 1 > W - 1 ? W - 1 : ix + 1);
+    int ya = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy), yb = iy + 1 < 0 ? 0 : (iy + 1 > H - 1 ? H - 1 : iy + 1);
+    vec_f32 r = vsmul_f32(x1, vsmul_f32(x0, screen_px(img, W, xb, yb)));
+    r = vadd_f32(r, vsmul_f32(y1, vsmul_f32(x0, screen_px(img, W, xb, ya))));
+    r = vadd_f32(r, vsmul_f32(y1, vsmul_f32(y0, screen_px(img, W, xa, ya))));
+    r = vadd_f32(r, vsmul_f32(x1, vsmul_f32(y0, screen_px(img, W, xa, yb))));
+    return r;
+}
+
+void orc_ssr(int W, int H, const tina_camera *cam, const tina_mesh *mesh, const tina_material *mat,
+             const int32_t *depth, const int32_t *occup, const float *norm, const float *coor, const float *image,
+             int nsamples, int nsteps, float stepsize, float tolerance, int blur, float *out)
+{
+    int size[TINA_MAX_OPS];
+    for (int i = 0; i < mat->nops; i++) {
+        int op = mat->ops[i].op;
+        if (op <= TINA_OP_EMISSION) size[i] = 1;
+        else if (op == TINA_OP_SCALE) size[i] = 1 + size[i - 1];
+        else size[i] = 1 + size[i - 1] + size[i - 1 - size[i - 1]];
+    }
+    const float resx = (float)W, resy = (float)H;
+    for (long P = 0; P < (long)W * H; P++) {
+        if (occup[P] < 0) continue;
+        uint32_t fu = (uint32_t)occup[P] - mesh->id_base;
+        if (fu >= (uint32_t)mesh->nfaces) continue;
+        int x = (int)(P % W), y = (int)(P / W);
+        vec_f32 normal = screen_px(norm, W, x, y);
+        float *o = out + 4 * P;
+        if (vdot_f32(normal, normal) < 1e-6f) {
+            o[0] = o[1] = o[2] = o[3] = 0.0f;
+            continue;
+        }
+        vec_f32 tc = coor ? v3_f32(coor[2 * P], coor[2 * P + 1], 0.0f) : vs_f32(0.0f);
+        float p_x = (float)x + cam->bias[0], p_y = (float)y + cam->bias[1];
+        vec_f32 vpos = v3_f32(((p_x / resx) * 2.0f) - 1.0f, ((p_y / resy) * 2.0f) - 1.0f,
+                              (float)depth[P] / 1073741824.0f);
+        vec_f32 pos = mapply_pos_f32(cam->V2W, vpos, NULL);
+        /* calc_viewdir (shader.py:81-93) */
+        vec_f32 ro0 = mapply_pos_f32(cam->V2W, v3_f32(vpos.x, vpos.y, -1.0f), NULL);
+        vec_f32 ro1 = mapply_pos_f32(cam->V2W, v3_f32(vpos.x, vpos.y, 1.0f), NULL);
+        vec_f32 viewdir = vneg_f32(vnormalized_f32(vsub_f32(ro1, ro0)));
+        vec_f32 pv[TINA_MAX_PARAMS];
+        eval_params_f32(mat, (long)fu, pos, normal, tc, vs_f32(1.0f), pv);
+        uint32_t rng = wang_noise((uint32_t)(y % blur) ^ wang_noise((uint32_t)(x % blur)));
+        float nst = (float)nsteps;
+        vec_f32 vpz = mapply_pos_f32(cam->W2V, pos, NULL);
+        float vtol = tolerance * (mapply_pos_f32(cam->W2V, vsub_f32(pos, vdivs_f32(viewdir, nst)), NULL).z - vpz.z);
+        float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < nsamples; i++) {
+            vec_f32 odir, wei;
+            sample_node(mat, pv, mat->nops - 1, size, viewdir, normal, &rng, &odir, &wei);
+            float c = vdot_f32(odir, viewdir);
+            float step = stepsize / (sqrtf(1.0f - c * c) * nst);
+            float jit = wang_random(&rng);
+            vec_f32 ro = vadd_f32(pos, vsmul_f32(step, vsmul_f32(jit, odir)));
+            for (int j = 0; j < nsteps; j++) {
+                ro = vadd_f32(ro, vsmul_f32(step, odir));
+                vec_f32 vro = mapply_pos_f32(cam->W2V, ro, NULL);
+                if (!((-1.0f <= vro.x) && (vro.x <= 1.0f) && (-1.0f <= vro.y) && (vro.y <= 1.0f) &&
+                      (-1.0f <= vro.z) && (vro.z <= 1.0f)))
+                    break;
+                float Dx = (vro.x * 0.5f + 0.5f) * resx, Dy = (vro.y * 0.5f + 0.5f) * resy;
+                int ix = (int)Dx, iy = (int)Dy;
+                if (ix > W - 1) ix = W - 1;
+                if (iy > H - 1) iy = H - 1;
+                float d = (float)depth[(long)iy * W + ix] / 1073741824.0f;
+                if ((vro.z - vtol < d) && (d < vro.z)) {
+                    vec_f32 clr = vmul_f32(screen_bilerp(image, W, H, Dx, Dy), wei);
+                    r[0] = r[0] + clr.x;
+                    r[1] = r[1] + clr.y;
+                    r[2] = r[2] + clr.z;
+                    r[3] = r[3] + 1.0f;
+                    break;
+                }
+            }
+        }
+        for (int k = 0; k < 4; k++) o[k] = r[k] / (float)nsamples;
+    }
+}
+
+void orc_ssr_apply(int W, int H, float *img, const float *buf, int blur)
+{
+    const int offs = blur / 2;
+    for (long P = 0; P < (long)W * H; P++) {
+        int x = (int)(P % W), y = (int)(P / W);
+        float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < blur; k++)
+            for (int l = 0; l < blur; l++) {
+                int xx = x + k - offs, yy = y + l - offs;
+                xx = xx < 0 ? 0 : (xx > W - 1 ? W - 1 : xx);
+                yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
+                const float *b = buf + 4 * ((long)yy * W + xx);
+                for (int c = 0; c < 4; c++) r[c] = r[c] + b[c];
+            }
+        for (int c = 0; c < 4; c++) r[c] = r[c] / (float)(blur * blur);
+        float m = 1.0f - r[3];
+        for (int c = 0; c < 3; c++) img[3 * P + c] = img[3 * P + c] * m + r[c];
+    }
+}
+
 /* ---------------------------------------------------------------- mesh nodes */
 
 /* mapply_pos (tina/common.py:169-183): res = mat[i][3] * 1, then res += mat[i][j] *

@@ -100,6 +100,16 @@ void orc_bake_ggx(const float *env, int ew, int eh, float *out, int ow, int oh,
  *   orc_smooth_normals  MeshSmoothNormal._update_normal (norm.py:35-47) with the
  *                       faces visited in ascending order; faces [nf][3] vertex
  *                       indices, positions taken from the face soup [nf][3][3] */
+/* SSR.render / SSR.apply (tina/postp/ssr.py:31-103), binary32, for the pixels
+ * whose id (occup) lies in the mesh's range; depth/occup [H][W] i32, norm
+ * [H][W][3], coor [H][W][2] or NULL, image [H][W][3]; out [H][W][4].  The
+ * non-TAA mode: WangHashRNG(P % blur).  orc_ssr_apply: the blur x blur box at
+ * offset -blur / 2 (clamped), img = img * (1 - a) + rgb. */
+void orc_ssr(int W, int H, const tina_camera *cam, const tina_mesh *mesh, const tina_material *mat,
+             const int32_t *depth, const int32_t *occup, const float *norm, const float *coor, const float *image,
+             int nsamples, int nsteps, float stepsize, float tolerance, int blur, float *out);
+void orc_ssr_apply(int W, int H, float *img, const float *buf, int blur);
+
 void orc_mesh_transform(const float trans[16], const float trans_normal[9], const float *verts, float *out_verts,
                         long nverts, const float *norms, float *out_norms, long nnorms);
 void orc_face_normals(const float *soup, long nfaces, float *out);

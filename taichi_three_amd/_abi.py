@@ -129,6 +129,9 @@ EXPORTS = [
     ('tina_gbuffer', _i32, [_vp, ctypes.POINTER(Mesh), _u32, _i32, _i32, _vp, ctypes.c_float, _vp, _vp]),
     ('tina_ssao', _i32, [_vp, _vp, _vp, _i32, _vp, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp]),
     ('tina_ssao_apply', _i32, [_vp, _i32, _i32, _vp, _i32, _i32, _vp]),
+    ('tina_ssr', _i32, [_vp, ctypes.POINTER(Mesh), ctypes.POINTER(Material), _vp, _vp, _vp, _i32, _i32,
+                        ctypes.c_float, ctypes.c_float, _i32, _i32, _u32, _vp, _vp]),
+    ('tina_ssr_apply', _i32, [_vp, _i32, _i32, _vp, _i32, _i32, _vp]),
     ('tina_shade', _i32, [_vp, ctypes.POINTER(Mesh), ctypes.POINTER(Material), ctypes.POINTER(Lights),
                           ctypes.POINTER(IBLTables), _vp, ctypes.POINTER(ctypes.c_float * 3), _u32, _vp]),
     ('tina_fill', _i32, [_vp, _vp, ctypes.POINTER(ctypes.c_float * 3), _u32, _vp]),

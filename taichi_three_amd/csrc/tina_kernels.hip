@@ -22,6 +22,7 @@
 #include "nn_shade.h"
 #include "shade_kernels.h"
 #include "postp_kernels.h"
+#include "ssr_kernels.h"
 #include "particle_kernels.h"
 #include "wire_kernels.h"
 
@@ -592,6 +593,47 @@ int tina_ssao_apply(float* d_img, int width, int height, const float* d_ao, int 
     hipLaunchKernelGGL(k_ssao_apply, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_img,
                        d_ao, width, height, nrot, box ? 1 : 0);
   return hip_check(nullptr, hipGetLastError(), "k_ssao_apply");
+}
+
+int tina_ssr(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const float* d_norm, const float* d_coor,
+             const float* d_image, int nsamples, int nsteps, float stepsize, float tolerance, int blur, int taa,
+             uint32_t seed, float* d_out, void* stream) {
+  if (!c) return fail(nullptr, TINA_ERR_INVALID, "ctx is NULL");
+  if (!mesh || !d_norm || !d_image || !d_out) return fail(c, TINA_ERR_INVALID, "tina_ssr: buffer is NULL");
+  if (mesh->nfaces < 0) return fail(c, TINA_ERR_INVALID, "nfaces must be >= 0");
+  if (nsamples < 1 || nsteps < 1 || blur < 1) return fail(c, TINA_ERR_INVALID, "tina_ssr: nsamples, nsteps, blur >= 1");
+  int rc = check_material(c, mat);
+  if (rc) return rc;
+  if (mesh->nfaces == 0) return TINA_OK;
+  ShadeArgs a;
+  if ((rc = make_shade_args(c, mesh, mat, nullptr, nullptr, nullptr, 0, a))) return rc;
+  if ((rc = flush_clear(c, (hipStream_t)stream))) return rc;
+  SsrArgs sa;
+  sa.norm = d_norm;
+  sa.coor = d_coor;
+  sa.image = d_image;
+  sa.out = d_out;
+  sa.nsamples = nsamples;
+  sa.nsteps = nsteps;
+  sa.blur = blur;
+  sa.taa = taa ? 1 : 0;
+  sa.stepsize = stepsize;
+  sa.tolerance = tolerance;
+  sa.seed = seed;
+  const long npix = (long)c->W * c->H;
+  hipLaunchKernelGGL(k_ssr, dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, a,
+                     (const uint64_t*)c->keys, sa);
+  TINA_LAUNCH(c, "k_ssr");
+  return TINA_OK;
+}
+
+int tina_ssr_apply(float* d_img, int width, int height, const float* d_ssr, int blur, int taa, void* stream) {
+  if (!d_img || !d_ssr) return fail(nullptr, TINA_ERR_INVALID, "tina_ssr_apply: buffer is NULL");
+  if (width < 1 || height < 1 || blur < 1) return fail(nullptr, TINA_ERR_INVALID, "tina_ssr_apply: bad size");
+  const long n = (long)width * height;
+  hipLaunchKernelGGL(k_ssr_apply, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, (hipStream_t)stream, d_img, d_ssr,
+                     width, height, blur, taa ? 1 : 0);
+  return hip_check(nullptr, hipGetLastError(), "k_ssr_apply");
 }
 
 int tina_shade(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, const tina_lights* lights,

@@ -6,7 +6,7 @@ import ctypes
 
 from . import _abi
 
-__all__ = ['ToneMapping', 'Blooming', 'FXAA', 'Accumator', 'SSAO']
+__all__ = ['ToneMapping', 'Blooming', 'FXAA', 'Accumator', 'SSAO', 'SSR']
 
 
 class ToneMapping:
@@ -182,3 +182,56 @@ class SSAO:
         t, W, H = _dims(image)
         _abi.check(_abi.load().tina_ssao_apply(_abi.ptr(t), W, H, _abi.ptr(self.img), int(self.noise_size),
                                                0 if self.taa else 1, _abi.stream_handle()))
+
+
+class SSR:
+    """Screen-space reflections (tina/postp/ssr.py:6-103) from the depth buffer,
+    a normal buffer (NormalShader) and, with texturing, a texcoord buffer
+    (TexcoordShader).  render() runs one pass per object over the pixels it won,
+    drawing reflection directions from that object's material (the reference
+    looks the material up in a per-pixel material-id buffer; the object that won
+    a pixel is the one whose material that buffer holds).  The non-TAA mode
+    draws the reference's own WangHashRNG(P % blurring) stream; with taa a
+    stream hashed from the pixel and the frame, where the reference draws
+    ti.random."""
+
+    def __init__(self, res, norm, coor=None, taa=False):
+        import torch
+        self.res = (int(res[0]), int(res[1]))
+        W, H = self.res
+        self.norm = norm
+        self.coor = coor
+        self.taa = bool(taa)
+        self.nsamples = 32 if not self.taa else 12
+        self.nsteps = 32 if not self.taa else 64
+        self.stepsize = 2.0
+        self.tolerance = 15.0
+        self.blurring = 4
+        self.frame = 0
+        self.img = torch.zeros((H, W, 4), dtype=torch.float32,
+                               device=torch.device('cuda', torch.cuda.current_device()))
+
+    def begin(self):
+        """Empties the reflection buffer for a new frame (render's pixels of no object)."""
+        self.img.zero_()
+        self.frame += 1
+
+    def render_object(self, engine, image, raster, material):
+        """render (ssr.py:51-56) over the pixels the object bound to `raster`
+        (set_object) won, drawing from `material`."""
+        if raster.nfaces == 0 or not hasattr(raster, 'mesh_struct'):
+            return                           # wires write no material id (IShader.blend_color)
+        t, _, _ = _dims(image)
+        lo = raster.lowered(material)
+        mesh = raster.mesh_struct()
+        _abi.check(engine._lib.tina_ssr(
+            engine._ctx, ctypes.byref(mesh), ctypes.byref(lo.struct), _abi.ptr(self.norm.storage),
+            _abi.ptr(self.coor.storage) if self.coor is not None else None, _abi.ptr(t), int(self.nsamples),
+            int(self.nsteps), ctypes.c_float(self.stepsize), ctypes.c_float(self.tolerance), int(self.blurring),
+            1 if self.taa else 0, self.frame & 0xffffffff, _abi.ptr(self.img), _abi.stream_handle()), engine._ctx)
+
+    def apply(self, image):
+        """apply (ssr.py:31-45): image * (1 - a) + rgb of the box-filtered buffer."""
+        t, W, H = _dims(image)
+        _abi.check(_abi.load().tina_ssr_apply(_abi.ptr(t), W, H, _abi.ptr(self.img), int(self.blurring),
+                                              1 if self.taa else 0, _abi.stream_handle()))

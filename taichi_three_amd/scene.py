@@ -14,15 +14,13 @@ from .engine import Engine, MAX
 from .field import Field
 from .lighting import Lighting, SkyboxLighting
 from .matr import Diffuse
-from .postp import ToneMapping, Blooming, FXAA, Accumator, SSAO
+from .postp import ToneMapping, Blooming, FXAA, Accumator, SSAO, SSR
 from .raster import TriangleRaster
 from .particle import ParticleRaster
 from .wireframe import WireframeRaster
-from .shader import Shader, ShaderGroup, NormalShader
+from .shader import Shader, ShaderGroup, NormalShader, TexcoordShader
 
 __all__ = ['Scene']
-
-_UNSUPPORTED = ('ssr',)
 
 
 class _Obj:
@@ -36,9 +34,6 @@ class _Obj:
 
 class Scene:
     def __init__(self, res_x=512, res_y=None, **options):
-        for k in _UNSUPPORTED:
-            if options.get(k, False):
-                raise NotImplementedError(f'Scene({k}=True): screen-space reflections are not provided by taichi_three_amd')
         self.options = options
         self.engine = Engine(res_x, res_y, maxfaces=options.get('maxfaces', MAX))
         self.res = self.engine.res
@@ -76,15 +71,25 @@ class Scene:
         # temporal anti-aliasing: a jittered pixel centre per frame, averaged (raster.py:83-84, 173-174, 206-214)
         self.taa = bool(options.get('taa', False))
         self.accum = Accumator(self.res) if self.taa else None
-        # SSAO reads a normal buffer that a pre-shader fills per object (raster.py:52-69)
+        # SSAO and SSR read a normal buffer that a pre-shader fills per object, SSR
+        # also a texcoord buffer when texturing (raster.py:42-69)
         self.ssao = None
-        if options.get('ssao', False):
+        self.ssr = None
+        if options.get('ssao', False) or options.get('ssr', False):
             self.norm_buffer = Field(torch.zeros((H, W, 3), dtype=torch.float32, device=self.engine.device))
             self.norm_shader = NormalShader(self.norm_buffer)
             self.pre_shaders.append(self.norm_shader)
+        if options.get('ssr', False):
+            self.coor_buffer = None
+            if options.get('texturing', False):
+                self.coor_buffer = Field(torch.zeros((H, W, 2), dtype=torch.float32, device=self.engine.device))
+                self.pre_shaders.append(TexcoordShader(self.coor_buffer))
+        if options.get('ssao', False):
             self.ssao = SSAO(self.res, self.norm_buffer, taa=self.taa)
-        if (self.blooming is not None or self.ssao is not None) and self.neural is not None:
-            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming or SSAO')
+        if options.get('ssr', False):
+            self.ssr = SSR(self.res, self.norm_buffer, self.coor_buffer, taa=self.taa)
+        if (self.blooming is not None or self.ssao is not None or self.ssr is not None) and self.neural is not None:
+            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming, SSAO or SSR')
 
     def _ensure_material_shader(self, material):
         if material in self.materials:
@@ -153,17 +158,25 @@ class Scene:
         mark('end')
 
     def _shade_all(self, order):
-        # tone mapping is fused into the shade kernels unless SSAO or blooming has to see the linear image
-        linear = self.blooming is not None or self.ssao is not None
+        # tone mapping is fused into the shade kernels unless SSAO, SSR or blooming
+        # has to see the linear image
+        linear = self.blooming is not None or self.ssao is not None or self.ssr is not None
         tm = _abi.TONEMAP if self.tonemap and not linear else 0
         if linear and self.neural is not None:
-            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming or SSAO')
+            raise ValueError('the learned shading writes tone-mapped colour: it cannot run before blooming, SSAO or SSR')
         self._shade_objects(order, tm)
-        if self.ssao is not None:               # raster.py:189-191, then the background (:197-198)
+        if self.ssao is not None:               # raster.py:189-191
             self.ssao.render(self.engine)
             self.ssao.apply(self.image)
-            if isinstance(self.lighting, SkyboxLighting):
-                self._fill_background(0)        # the sky is drawn after the occlusion, over empty pixels
+        if self.ssr is not None:                # raster.py:193-195
+            self.ssr.begin()
+            for obj, info in order:
+                r = info.raster
+                r._fb, r._mesh, r.nfaces, r.id_base = info.fb, obj, info.nfaces, info.id_base
+                self.ssr.render_object(self.engine, self.image, r, info.material)
+            self.ssr.apply(self.image)
+        if (self.ssao is not None or self.ssr is not None) and isinstance(self.lighting, SkyboxLighting):
+            self._fill_background(0)            # the sky is drawn after them, over empty pixels (:197-198)
         if self.blooming is not None:
             self.blooming.apply(self.image)
         if self.tonemap and linear:
