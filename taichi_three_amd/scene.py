@@ -96,7 +96,10 @@ class Scene:
             return
         shader = Shader(self.image, self.lighting, material)
         self.materials.append(material)
-        self.shaders[material] = ShaderGroup(self.pre_shaders + [shader] + self.post_shaders)
+        # the pre-shaders (normal and texcoord buffers) run right after each
+        # object's coverage pass instead (render), as the reference's per-object
+        # loop leaves them
+        self.shaders[material] = ShaderGroup([shader] + self.post_shaders)
 
     def add_object(self, object, material=None, raster=None):
         """Adds a mesh, a wire mesh or a particle set; the raster is picked by duck typing (raster.py:112-146)."""
@@ -150,6 +153,12 @@ class Scene:
             r.set_object(obj)
             info.id_base, info.nfaces, info.fb = r.id_base, r.nfaces, r._fb
             r.render_occup()
+            # the reference runs the pre-shaders inside each object's render_color
+            # (raster.py:183-187, 95-110): they write the pixels the object wins at
+            # that point, and a wire drawn over them later (which only blends its
+            # colour) leaves what they wrote
+            for s in self.pre_shaders:
+                s.post_shade(r, r.id_base, r.nfaces)
             order.append((obj, info))
         mark('shade')
         self._shade_all(order)

@@ -65,3 +65,51 @@ def test_all_passes_with_taa_average(gpu):
     acc = scene.img.to_numpy()
     assert scene.accum.count == 3 and np.isfinite(acc).all()
     assert np.abs(acc - np.mean(frames, axis=0)).max() <= 1e-5
+
+
+def test_normals_under_the_wire_cage_follow_the_reference(gpu):
+    """The normal buffer and the occlusion of the mesh + wire cage + particles
+    scene against the oracle's per-object construction: in the reference each
+    object's render_color runs the NormalShader on the pixels the object wins at
+    its turn, and a wire drawn later only blends its colour, so a wire pixel keeps
+    the normal of the surface it was drawn over (raster.py:183-187,
+    wireframe.py:88-95, shader.py:9-18)."""
+    import taichi_three_amd as tina
+    from taichi_three_amd import _abi
+    from taichi_three_amd.matr import lower_material
+    print('case: normals under a wire cage, ssao 320x240', flush=True)
+    scene, _ = _render(ssao=True)
+    eng = scene.engine
+    W, H = int(scene.res[0]), int(scene.res[1])
+    cam = orc.camera(eng.W2V, eng.V2W, eng.bias)
+    lt = scene.lighting.to_struct()
+    lamp = lower_material(tina.Lamp('normal'), dict(attrs={})).struct
+    sphere, cage, pars = list(scene.objects)
+    od = np.full((H, W), 2 ** 30, np.int32)
+    oi = np.full((H, W), -1, np.int32)
+    norm = np.zeros((H, W, 3), np.float32)
+    fb = sphere.face_buffers()
+    sv, sn = fb['verts'].cpu().numpy(), fb['norms'].cpu().numpy()
+    od, oi = orc.raster(W, H, cam, sv, id_base=0, flags=_abi.CULL | _abi.CLIP, depth=od, occup=oi)
+    norm = orc.shade(W, H, cam, orc.HostMesh(sv, sn), lamp, lt, oi, _abi.SMOOTH)
+    base = len(sv)
+    wv = cage.face_buffers()['verts'].cpu().numpy()
+    wflags = _abi.CLIP if scene.wireframe_raster.clipping else 0
+    od, oi = orc.raster_wires(W, H, cam, wv, id_base=base, flags=wflags, depth=od, occup=oi)
+    wire_px = (oi >= base) & (oi < base + len(wv))
+    base += len(wv)
+    pb = pars.particle_buffers()
+    pp, pr = pb['pos'].cpu().numpy(), pb['radius'].cpu().numpy()
+    od, oi = orc.raster_particles(W, H, cam, pp, pr, id_base=base, flags=_abi.CLIP, depth=od, occup=oi)
+    mine = oi >= base
+    pn = orc.shade(W, H, cam, orc.particle_mesh(pp, pr, id_base=base), lamp, lt, oi, _abi.PARTICLES)
+    norm = np.where(mine[..., None], pn, norm)
+    assert np.array_equal(eng.ids.to_numpy().T, oi)
+    got = scene.norm_buffer.storage.cpu().numpy()
+    over = wire_px & (np.abs(norm).sum(axis=2) > 0)
+    assert over.sum() > 50                   # wire pixels drawn over the sphere
+    assert np.abs(got - norm).max() <= 1e-6
+    ao = opp.ssao(eng.W2V, eng.V2W, eng.bias, od, norm, scene.ssao.samples.cpu().numpy(),
+                  scene.ssao.rotations.cpu().numpy(), radius=scene.ssao.radius, thresh=scene.ssao.thresh,
+                  factor=scene.ssao.factor)
+    assert np.abs(scene.ssao.img.cpu().numpy() - ao).max() <= 1e-6
