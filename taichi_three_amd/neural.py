@@ -138,6 +138,32 @@ class NeuralShading:
         self.pack()
         return self
 
+    # ---- checkpoint ---------------------------------------------------------------
+    def save(self, path):
+        """Writes the float32 master weights, both Adam moments, the step count
+        and the settings to an .npz file (arrays only: loading it executes
+        nothing, where the reference pickles its fields, tina/shield.py:7-57)."""
+        np.savez(path, params=self.params.detach().cpu().numpy(), m1=self.m1.detach().cpu().numpy(),
+                 m2=self.m2.detach().cpu().numpy(), step=np.array(self.step),
+                 specular=np.array(self.specular), threshold=np.array(self.threshold))
+
+    @classmethod
+    def load(cls, path, lighting):
+        """A surrogate for `lighting` resumed from `save`: training continues
+        from the saved moments and step, and the packed weights are ready."""
+        torch = _torch()
+        with np.load(path, allow_pickle=False) as z:
+            arrays = {k: z[k] for k in ('params', 'm1', 'm2')}
+            for k, v in arrays.items():
+                if v.shape != (NPARAMS,) or v.dtype != np.float32:
+                    raise ValueError(f'{k}: expected {NPARAMS} float32 values, got {v.shape} {v.dtype}')
+            self = cls(lighting, specular=float(z['specular']), threshold=float(z['threshold']))
+            self.step = int(z['step'])
+        for k, v in arrays.items():
+            getattr(self, k).copy_(torch.from_numpy(v))
+        self.pack()
+        return self
+
     # ---- inference --------------------------------------------------------------
     def pack(self):
         _abi.check(_abi.load().tina_mlp_pack(_abi.ptr(self.params), _abi.ptr(self.packed), _abi.stream_handle()))

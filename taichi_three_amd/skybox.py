@@ -216,6 +216,48 @@ class IBLTables:
             nsamples = int(nsamples * 3 ** (step * 4))
         self.struct = make_ibl_struct(self.diff, self.spec)
 
+    def save(self, path):
+        """Writes the baked maps (the Lambert map and levels 1-6; level 0 is the
+        skybox itself) and the precision to an .npz file.  The reference
+        pickles its Taichi fields for the same purpose (tina/shield.py:7-57);
+        an .npz holds only arrays, so loading it executes nothing."""
+        arrays = {'precision': np.array(self.precision), 'env_shape': np.array(self.skybox.img.shape),
+                  'diff': self.diff.detach().cpu().numpy()}
+        for i, m in enumerate(self.spec[1:], 1):
+            arrays[f'spec{i}'] = m.detach().cpu().numpy()
+        np.savez(path, **arrays)
+
+    @classmethod
+    def load(cls, path, skybox):
+        """The tables of `save`, on `skybox`'s device, without baking.  The file
+        must have been written for a skybox of the same shape."""
+        torch = _torch()
+        with np.load(path, allow_pickle=False) as z:
+            if tuple(int(v) for v in z['env_shape']) != tuple(skybox.img.shape):
+                raise ValueError(f'tables baked for a {tuple(z["env_shape"])} skybox, not {tuple(skybox.img.shape)}')
+            self = cls.__new__(cls)
+            self.skybox = skybox
+            self.precision = int(z['precision'])
+            self.levels = rough_levels()
+            dev = skybox.img.device
+            res = skybox.resolution // 6
+            self.diff = torch.from_numpy(np.ascontiguousarray(z['diff'], np.float32)).to(dev)
+            self.diff_samples = 128 * self.precision
+            if tuple(self.diff.shape) != (2 * res, res, 3):
+                raise ValueError(f'Lambert map of shape {tuple(self.diff.shape)}, expected {(2 * res, res, 3)}')
+            self.spec, self.spec_samples = [skybox.img], [0]
+            resolution, nsamples = skybox.resolution, 8 * self.precision
+            for i, (rough, step) in enumerate(self.levels, 1):
+                resolution = int(resolution / 2 ** (step * 4))
+                lvl = torch.from_numpy(np.ascontiguousarray(z[f'spec{i}'], np.float32)).to(dev)
+                if tuple(lvl.shape) != (2 * resolution, resolution, 3):
+                    raise ValueError(f'level {i} of shape {tuple(lvl.shape)}, expected {(2 * resolution, resolution, 3)}')
+                self.spec.append(lvl)
+                self.spec_samples.append(nsamples)
+                nsamples = int(nsamples * 3 ** (step * 4))
+        self.struct = make_ibl_struct(self.diff, self.spec)
+        return self
+
     def __getitem__(self, key):
         """ibltab['env' | 'diff' | 'spec' | 'lut'] as the reference's dict has them."""
         return {'env': self.skybox, 'diff': self.diff, 'spec': tuple(self.spec), 'lut': LUT_CORNER}[key]
