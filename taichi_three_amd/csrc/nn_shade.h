@@ -7,7 +7,8 @@
 // takes the list 64 entries at a time: network inputs per lane, the network on
 // four 16-pixel MFMA blocks, the ACES colour stored where the error estimate is
 // at most `threshold` and the pixel queued for the analytic kernel otherwise
-// (per chunk in LDS, then one global atomic reserves its queue segment).
+// (per chunk in LDS, then one global atomic reserves its queue segment);
+// pixels with a NaN or infinite feature (degenerate faces) are queued too.
 //
 // The inputs are computed with v_rcp / v_rsq in place of correctly rounded
 // division and square root: the surrogate's colour is approximate by design,
@@ -150,50 +151,85 @@ __device__ __forceinline__ uint32_t laundered_lane(int lane) {
   return l;
 }
 
-// The network on n feature rows X[n][12] -> O[n][4] (max(0, colour), error
-// estimate), with k_shade_nn's arithmetic: persistent workgroups, each wave
-// taking 64 rows at a time.  The B operand of layer 1 is read straight from the
-// rows (lane (g, j) of block nb holds features 8g..8g+7 of row 16 nb + j: two
-// 16-byte loads for g = 0, one for g = 1, zeros above feature 11) and the lanes
-// of group 0 store the outputs of their rows, so nothing passes through LDS
-// but the biases.
-__global__ void __launch_bounds__(kBlock)
-k_mlp_rows(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, float* __restrict__ O) {
-  __shared__ __attribute__((aligned(16))) float s_b[3 * 64 + 16];
-  __shared__ __attribute__((aligned(16))) bf16x8 s_w[kNnFrags * 64];
-  const int lane = __lane_id(), wv = threadIdx.x / 64, g = lane / 16, j = lane % 16;
-  nn_stage(pk, s_w, s_b);
-  __syncthreads();
-  const long nchunks = (n + 63) / 64;
-  for (long c = (long)blockIdx.x * (kBlock / 64) + wv; c < nchunks; c += (long)gridDim.x * (kBlock / 64)) {
-    bf16x8 xb[4];
-    f32x4 ob[4];
+// Raw buffer resource over `bytes` bytes at p (gfx9 layout, no stride): loads
+// at offsets outside the range return 0 and stores there are dropped, so a lane
+// with nothing to move needs no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t nn_rsrc(const void* p, long bytes) {
+  const int nr = bytes <= 0 ? 0 : bytes >= 0x7fffffffL ? 0x7fffffff : (int)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, nr, 0x00020000);
+}
+
+// Layer 1's B operands of rows [64c, 64c + 64) of X, raw: lane (g, j) of block nb
+// takes features 8g..8g+7 of row 16 nb + j -- two 16-byte loads for g = 0, one
+// (features 8-11, then zeros) for g = 1, zeros for g >= 2 and rows past n.
+__device__ __forceinline__ void nn_rows_load(const float* __restrict__ X, long n, long c, int g, int j, u32x4 (&p)[4],
+                                             u32x4 (&q)[4]) {
+  const long r0 = c * 64;
+  const __amdgpu_buffer_rsrc_t R = nn_rsrc(X + r0 * kNnIn, (n - r0) * (kNnIn * 4));
+  constexpr uint32_t kOut = 0x80000000u;  // past any range: loads 0
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) {
-      const long r = c * 64 + 16 * nb + j;
-      const float4* row = (const float4*)(X + r * kNnIn);
-      float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q = p;
-      if (r < n && g < 2) {
-        p = row[2 * g];
-        if (g == 0) q = row[1];
-      }
-      const u32x4 v = {bf16_pair(p.x, p.y), bf16_pair(p.z, p.w), bf16_pair(q.x, q.y), bf16_pair(q.z, q.w)};
-      xb[nb] = __builtin_bit_cast(bf16x8, v);
-    }
-    nn_eval64(s_w, laundered_lane(lane), s_b, g, xb, ob);
-    if (g == 0) {
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++) {
-        const long r = c * 64 + 16 * nb + j;
-        if (r < n)
-          *(float4*)(O + r * 4) = make_float4(fmaxf(ob[nb][0], 0.0f), fmaxf(ob[nb][1], 0.0f), fmaxf(ob[nb][2], 0.0f),
-                                              ob[nb][3]);
-      }
-    }
+  for (int nb = 0; nb < 4; nb++) {
+    const uint32_t row = (uint32_t)(16 * nb + j) * (kNnIn * 4);
+    p[nb] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(R, g < 2 ? row + 32u * g : kOut, 0, 0));
+    q[nb] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(R, g == 0 ? row + 16u : kOut, 0, 0));
   }
 }
 
-__global__ void __launch_bounds__(kBlock)
+// The network on n feature rows X[n][12] -> O[n][4] (max(0, colour), error
+// estimate), with k_shade_nn's arithmetic: persistent workgroups, each wave
+// taking 64 rows at a time.  Each wave loads its next rows before it evaluates
+// the current ones; the loads and stores are raw buffer operations whose
+// out-of-range lanes read zeros and write nothing, so the loop has no branch
+// around them and the wait before the next rows leaves the stores in flight.
+// Three waves per SIMD: with that bound the MFMAs accumulate in VGPRs (under a
+// one-wave bound the compiler keeps their results in AGPRs and copies every
+// value out with v_accvgpr_read before the activation).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_mlp_rows(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, float* __restrict__ O) {
+  __shared__ __attribute__((aligned(16))) float s_b[3 * 64 + 16];
+  __shared__ __attribute__((aligned(16))) bf16x8 s_w[kNnFrags * 64];
+  const int lane = __lane_id(), g = lane / 16, j = lane % 16;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);  // uniform: the buffer ranges live in SGPRs
+  nn_stage(pk, s_w, s_b);
+  __syncthreads();
+  const long nchunks = (n + 63) / 64;
+  const long stride = (long)gridDim.x * (kBlock / 64);
+  long c = (long)blockIdx.x * (kBlock / 64) + wv;
+  u32x4 p[4], q[4];
+  nn_rows_load(X, n, c, g, j, p, q);
+  {
+    // as many stores (to an empty range: dropped) as every pass issues after its
+    // loads, so the loop entry leaves the same operations in flight as the back
+    // edge and the wait at the top of the loop covers the loads only
+    const __amdgpu_buffer_rsrc_t W0 = nn_rsrc(O, 0);
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++)
+      __builtin_amdgcn_raw_buffer_store_b128((u32x4){0u, 0u, 0u, 0u}, W0, (uint32_t)(16 * nb + j) * 16u, 0, 0);
+  }
+  do {  // the grid gives every wave at least one chunk; a wave past the end loads and stores nothing
+    bf16x8 xb[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const f32x4 a = __builtin_bit_cast(f32x4, p[nb]), b = __builtin_bit_cast(f32x4, q[nb]);
+      const u32x4 v = {bf16_pair(a[0], a[1]), bf16_pair(a[2], a[3]), bf16_pair(b[0], b[1]), bf16_pair(b[2], b[3])};
+      xb[nb] = __builtin_bit_cast(bf16x8, v);
+    }
+    nn_rows_load(X, n, c + stride, g, j, p, q);
+    f32x4 ob[4];
+    nn_eval64(s_w, laundered_lane(lane), s_b, g, xb, ob);
+    const long r0 = c * 64;
+    const __amdgpu_buffer_rsrc_t W = nn_rsrc(O + r0 * kNnOut, (n - r0) * (kNnOut * 4));
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const f32x4 v = {fmaxf(ob[nb][0], 0.0f), fmaxf(ob[nb][1], 0.0f), fmaxf(ob[nb][2], 0.0f), ob[nb][3]};
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), W,
+                                             g == 0 ? (uint32_t)(16 * nb + j) * 16u : 0x80000000u, 0, 0);
+    }
+    c += stride;
+  } while (c < nchunks);
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const __bf16* __restrict__ pk,
            float threshold, uint32_t* __restrict__ fbq, long segcap, uint32_t* __restrict__ fbn,
            float* __restrict__ img) {
@@ -261,6 +297,7 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
       float x[kNnIn];
 #pragma unroll
       for (int k = 0; k < kNnIn; k++) x[k] = 0.0f;
+      bool finite = true;  // a NaN or infinite feature (a degenerate face) goes to the analytic kernel
       if (live) {
         P = base + s_pix[q];
         const uint32_t f = s_face[q];
@@ -283,6 +320,10 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
         x[7] = __logf(rough);
         x[8] = metal;
         x[9] = albedo.x; x[10] = albedo.y; x[11] = albedo.z;
+        float sum = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kNnIn; k++) sum += x[k] * 0.0f;  // NaN iff some feature is NaN or infinite
+        finite = sum == 0.0f;
       }
       u32x4* row = (u32x4*)s_x[wv][lane];
       row[0] = (u32x4){bf16_pair(x[0], x[1]), bf16_pair(x[2], x[3]), bf16_pair(x[4], x[5]), bf16_pair(x[6], x[7])};
@@ -307,7 +348,7 @@ k_shade_nn(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const 
       wave_sync_lds();
       if (live) {
         const f32x4 o = *(const f32x4*)s_o[wv][lane];
-        const bool fall = !(o[3] <= threshold);
+        const bool fall = !(o[3] <= threshold) || !finite;
         const unsigned long long fm = __ballot(fall);
         uint32_t at = 0;
         if (lane == 0 && fm) at = atomicAdd(&s_fbn, (uint32_t)__popcll(fm));
