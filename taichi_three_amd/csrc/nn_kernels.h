@@ -26,7 +26,7 @@ namespace tina {
 constexpr int kNnIn = 12, kNnHid = 64, kNnOut = 4;
 constexpr int kOffW1 = 0, kOffB1 = 768, kOffW2 = 832, kOffB2 = 4928, kOffW3 = 4992, kOffB3 = 9088;
 constexpr int kOffW4 = 9152, kOffB4 = 9408, kNnParams = 9412;
-constexpr int kNnBatchPerBlock = 64;      // samples per k_mlp_grad workgroup (one wave)
+constexpr int kNnBatchPerBlock = 64;      // samples per k_mlp_grad workgroup (four waves)
 constexpr float kHuberBeta = 0.01f;
 constexpr float kErrWeight = 1.0f;
 
@@ -151,161 +151,253 @@ k_nn_analytic_rows(const ShadeArgs a_by_value, const float* __restrict__ X, long
 }
 
 // ---- training ------------------------------------------------------------------
-
-// One wave per workgroup, one sample per lane.  Every lane reads a weight at the
-// same address (uniform: scalar loads), the activations of the 64 samples live
-// in padded LDS rows (row stride 65: the lanes of a column access hit distinct
-// banks), and each delta overwrites the activation rows it no longer needs.
-// The workgroup's gradient -- summed over its samples in sample order -- goes
-// to row blockIdx.x of `partial`.
+//
+// One workgroup (four waves) per 64 samples; every product of the forward and
+// backward pass is a 16 x 16 tile of v_mfma_f32_16x16x4_f32 (float32 operands
+// and accumulation), lane l holding A[l % 16][l / 16], B[l / 16][l % 16] and
+// D[4 (l / 16) + q][l % 16], q = 0..3.  Activations live in LDS as one row per
+// sample (stride kGradRow floats: a row-block read of 16-byte vectors is
+// bank-conflict free), and each delta overwrites the activation rows it no
+// longer needs.  The K order inside a product is free as long as both operands
+// use it: layers 2-4 take input neuron c = 16 (l / 16) + kk at step kk, so a
+// lane reads its B values as four 16-byte vectors, and the weight gradients sum
+// over samples s = 16 (l / 16) + kk.  Each wave keeps the weights it multiplies
+// in registers, loaded once before the first product.  The workgroup's gradient
+// goes to row blockIdx.x of `partial`.
+//
 // Loss per sample: mean_c smoothL1(o_c - y_c) + kErrWeight/2 (o_3 - max_c|o_c - y_c|)^2,
 // averaged over the `batch` samples of the step; the error term trains W4[3], b4[3] only.
-__global__ void __launch_bounds__(64)
-k_mlp_grad(const float* __restrict__ params, const float* __restrict__ X, const float* __restrict__ Y,
-           long first, long nsamples, int batch, float* __restrict__ partial) {
-  __shared__ float sX[64 * 13];
-  __shared__ float sH1[64 * 65], sH2[64 * 65], sH3[64 * 65];
-  __shared__ float sGo[64 * 4];
-  const int t = threadIdx.x;
-  const long s = first + (long)blockIdx.x * kNnBatchPerBlock + t;
-  const bool valid = s < nsamples;
-  float x[kNnIn], y[3];
-#pragma unroll
-  for (int k = 0; k < kNnIn; k++) x[k] = valid ? X[s * kNnIn + k] : 0.0f;
-#pragma unroll
-  for (int c = 0; c < 3; c++) y[c] = valid ? Y[s * 4 + c] : 0.0f;
-#pragma unroll
-  for (int k = 0; k < kNnIn; k++) sX[t * 13 + k] = x[k];
-  __syncthreads();
 
-  // forward
-  for (int j = 0; j < kNnHid; j++) {
-    float z = params[kOffB1 + j];
-#pragma unroll
-    for (int k = 0; k < kNnIn; k++) z += params[kOffW1 + j * kNnIn + k] * x[k];
-    sH1[t * 65 + j] = fmaxf(z, 0.0f);
-  }
-  for (int j = 0; j < kNnHid; j++) {
-    float z = params[kOffB2 + j];
-    for (int k = 0; k < kNnHid; k++) z += params[kOffW2 + j * kNnHid + k] * sH1[t * 65 + k];
-    sH2[t * 65 + j] = fmaxf(z, 0.0f);
-  }
-  for (int j = 0; j < kNnHid; j++) {
-    float z = params[kOffB3 + j];
-    for (int k = 0; k < kNnHid; k++) z += params[kOffW3 + j * kNnHid + k] * sH2[t * 65 + k];
-    sH3[t * 65 + j] = fmaxf(z, 0.0f);
-  }
-  float o[kNnOut];
-#pragma unroll
-  for (int c = 0; c < kNnOut; c++) {
-    float z = params[kOffB4 + c];
-    for (int k = 0; k < kNnHid; k++) z += params[kOffW4 + c * kNnHid + k] * sH3[t * 65 + k];
-    o[c] = z;
-  }
-  // output gradient
-  const float inv_b = 1.0f / (float)batch;
-  float e = 0.0f;
-  float go[kNnOut];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const float d = o[c] - y[c];
-    e = fmaxf(e, fabsf(d));
-    const float hg = fabsf(d) < kHuberBeta ? d / kHuberBeta : (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f));
-    go[c] = valid ? hg * (inv_b / 3.0f) : 0.0f;
-  }
-  go[3] = valid ? kErrWeight * (o[3] - e) * inv_b : 0.0f;
-#pragma unroll
-  for (int c = 0; c < kNnOut; c++) sGo[t * 4 + c] = go[c];
-  __syncthreads();
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-  float* P = partial + (size_t)blockIdx.x * kNnParams;
-  // dW4[c][t] = sum_b go[b][c] h3[b][t]; db4
-  {
-    float acc[kNnOut] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int b = 0; b < 64; b++) {
-      const float h = sH3[b * 65 + t];
+constexpr int kGradRow = 68;  // LDS floats per sample row of an activation / delta matrix
+
+__device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Hout[s][16m + n] = relu(b[16m + n] + sum_c W[16m + n][c] Hin[s][c]) for the 64
+// samples (four 16-sample tiles), K = 64 in the order c = 16 k + kk.
+__device__ __forceinline__ void grad_dense(const float (&w)[16], const float* __restrict__ bias,
+                                           const float* Hin, float* Hout, int m, int i, int k) {
+  const f32x4 b0 = *(const f32x4*)(bias + 16 * m + 4 * k);
 #pragma unroll
-      for (int c = 0; c < kNnOut; c++) acc[c] += sGo[b * 4 + c] * h;
+  for (int sb = 0; sb < 4; sb++) {
+    const f32x4* row = (const f32x4*)(Hin + (16 * sb + i) * kGradRow + 16 * k);
+    f32x4 acc = b0;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const f32x4 r = row[v];
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc = mfma_f32(w[4 * v + e], r[e], acc);
     }
+    f32x4 h;
 #pragma unroll
-    for (int c = 0; c < kNnOut; c++) P[kOffW4 + c * kNnHid + t] = acc[c];
-    if (t < kNnOut) {
-      float sb = 0.0f;
-      for (int b = 0; b < 64; b++) sb += sGo[b * 4 + t];
-      P[kOffB4 + t] = sb;
-    }
-  }
-  // dW3[i][t] = sum_b d3[b][i] h2[b][t]; db3[t] = sum_b d3[b][t]
-  auto outer = [&](const float* D, const float* H, int offW, int offB) {
-    float acc[kNnHid];
-#pragma unroll
-    for (int i = 0; i < kNnHid; i++) acc[i] = 0.0f;
-    float sb = 0.0f;
-    for (int b = 0; b < 64; b++) {
-      const float h = H[b * 65 + t];
-      sb += D[b * 65 + t];
-#pragma unroll
-      for (int i = 0; i < kNnHid; i++) acc[i] += D[b * 65 + i] * h;
-    }
-#pragma unroll
-    for (int i = 0; i < kNnHid; i++) P[offW + i * kNnHid + t] = acc[i];
-    P[offB + t] = sb;
-  };
-  __syncthreads();
-  // delta3 = (W4[0:3]^T go) * (h3 > 0) -> this lane's row of sH3
-  for (int j = 0; j < kNnHid; j++) {
-    float z = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 3; c++) z += params[kOffW4 + c * kNnHid + j] * go[c];
-    sH3[t * 65 + j] = sH3[t * 65 + j] > 0.0f ? z : 0.0f;
-  }
-  __syncthreads();
-  outer(sH3, sH2, kOffW3, kOffB3);
-  __syncthreads();
-  // delta2 = (W3^T d3) * (h2 > 0) -> this lane's row of sH2
-  for (int j = 0; j < kNnHid; j++) {
-    float z = 0.0f;
-    for (int i = 0; i < kNnHid; i++) z += params[kOffW3 + i * kNnHid + j] * sH3[t * 65 + i];
-    sH2[t * 65 + j] = sH2[t * 65 + j] > 0.0f ? z : 0.0f;
-  }
-  __syncthreads();
-  outer(sH2, sH1, kOffW2, kOffB2);
-  __syncthreads();
-  // delta1 = (W2^T d2) * (h1 > 0) -> this lane's row of sH1
-  for (int j = 0; j < kNnHid; j++) {
-    float z = 0.0f;
-    for (int i = 0; i < kNnHid; i++) z += params[kOffW2 + i * kNnHid + j] * sH2[t * 65 + i];
-    sH1[t * 65 + j] = sH1[t * 65 + j] > 0.0f ? z : 0.0f;
-  }
-  __syncthreads();
-  // dW1[t][k] = sum_b d1[b][t] x[b][k]; db1[t]
-  {
-    float acc[kNnIn];
-#pragma unroll
-    for (int k = 0; k < kNnIn; k++) acc[k] = 0.0f;
-    float sb = 0.0f;
-    for (int b = 0; b < 64; b++) {
-      const float d = sH1[b * 65 + t];
-      sb += d;
-#pragma unroll
-      for (int k = 0; k < kNnIn; k++) acc[k] += d * sX[b * 13 + k];
-    }
-#pragma unroll
-    for (int k = 0; k < kNnIn; k++) P[kOffW1 + t * kNnIn + k] = acc[k];
-    P[kOffB1 + t] = sb;
+    for (int q = 0; q < 4; q++) h[q] = fmaxf(acc[q], 0.0f);
+    *(f32x4*)(Hout + (16 * sb + i) * kGradRow + 16 * m + 4 * k) = h;
   }
 }
 
-// g = sum over the `nrows` rows of partial (row order); Adam with bias
-// corrections c1 = 1/(1 - b1^t), c2 = 1/(1 - b2^t) supplied by the host.
+// D[s][16m + n] = (H[s][16m + n] > 0) * sum_c W[c][16m + n] Dn[s][c] (A = W^T from
+// registers), written over H.
+__device__ __forceinline__ void grad_delta(const float (&wt)[16], const float* Dn, float* H, int m, int i, int k) {
+#pragma unroll
+  for (int sb = 0; sb < 4; sb++) {
+    const f32x4* row = (const f32x4*)(Dn + (16 * sb + i) * kGradRow + 16 * k);
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const f32x4 r = row[v];
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc = mfma_f32(wt[4 * v + e], r[e], acc);
+    }
+    f32x4* hp = (f32x4*)(H + (16 * sb + i) * kGradRow + 16 * m + 4 * k);
+    const f32x4 h = *hp;
+    f32x4 d;
+#pragma unroll
+    for (int q = 0; q < 4; q++) d[q] = h[q] > 0.0f ? acc[q] : 0.0f;
+    *hp = d;
+  }
+}
+
+// dW[16n + j][16m + r] = sum_s D[s][16n + j] H[s][16m + r] for this wave's input
+// block m and the four output blocks n, written transposed from the D layout
+// (float4 per lane) to the weight's rows of `out` (row length kin), then the
+// bias gradient sum_s D[s][o] by lanes (wave 0 only: o = lane).
+__device__ __forceinline__ void grad_weights(const float* H, int hstride, const float* D, float* __restrict__ outW,
+                                             int kin, int m, int i, int k) {
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const int s = 16 * k + kk;
+      acc = mfma_f32(H[s * hstride + 16 * m + i], D[s * kGradRow + 16 * n + i], acc);
+    }
+    const int r = 16 * m + 4 * k;  // input neurons r..r+3 of output neuron 16n + i
+    if (r < kin) *(f32x4*)(outW + (16 * n + i) * kin + r) = acc;
+  }
+}
+
+__device__ __forceinline__ void grad_bias(const float* D, float* __restrict__ outB, int o) {
+  float sb = 0.0f;
+  for (int s = 0; s < 64; s++) sb += D[s * kGradRow + o];
+  outB[o] = sb;
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))  // MFMA results in VGPRs
+k_mlp_grad(const float* __restrict__ params, const float* __restrict__ X, const float* __restrict__ Y,
+           long first, long nsamples, int batch, float* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) float sX[64 * 16];   // [s][16]: features, 12..15 zero
+  __shared__ __attribute__((aligned(16))) float sH1[64 * kGradRow], sH2[64 * kGradRow], sH3[64 * kGradRow];
+  __shared__ __attribute__((aligned(16))) float sGo[64 * 16];  // [s][16]: output gradients, 4..15 zero
+  const int tid = threadIdx.x, lane = tid & 63, i = lane & 15, k = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const long s0 = first + (long)blockIdx.x * 64;
+  float* __restrict__ P = partial + (size_t)blockIdx.x * kNnParams;
+
+  // the weights this wave multiplies, in the K order of each product
+  float w1[3], w2[16], w3[16], w4[16], w4t[1], w3t[16], w2t[16];
+#pragma unroll
+  for (int kk = 0; kk < 3; kk++) w1[kk] = params[kOffW1 + (16 * w + i) * kNnIn + 4 * kk + k];
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++) {
+    w2[kk] = params[kOffW2 + (16 * w + i) * kNnHid + 16 * k + kk];
+    w3[kk] = params[kOffW3 + (16 * w + i) * kNnHid + 16 * k + kk];
+    w4[kk] = i < kNnOut ? params[kOffW4 + i * kNnHid + 16 * k + kk] : 0.0f;
+    w3t[kk] = params[kOffW3 + (16 * k + kk) * kNnHid + 16 * w + i];
+    w2t[kk] = params[kOffW2 + (16 * k + kk) * kNnHid + 16 * w + i];
+  }
+  w4t[0] = k < 3 ? params[kOffW4 + k * kNnHid + 16 * w + i] : 0.0f;  // the colour rows of W4 only
+
+  for (int e = tid; e < 64 * 16; e += 256) {
+    const int s = e >> 4, f = e & 15;
+    sX[e] = (f < kNnIn && s0 + s < nsamples) ? X[(s0 + s) * kNnIn + f] : 0.0f;
+    sGo[e] = 0.0f;
+  }
+  __syncthreads();
+
+  // forward: layer 1 (K = 12 in the order f = 4 kk + k), layers 2 and 3
+  {
+    const f32x4 b0 = *(const f32x4*)(params + kOffB1 + 16 * w + 4 * k);
+#pragma unroll
+    for (int sb = 0; sb < 4; sb++) {
+      f32x4 acc = b0;
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++) acc = mfma_f32(w1[kk], sX[(16 * sb + i) * 16 + 4 * kk + k], acc);
+      f32x4 h;
+#pragma unroll
+      for (int q = 0; q < 4; q++) h[q] = fmaxf(acc[q], 0.0f);
+      *(f32x4*)(sH1 + (16 * sb + i) * kGradRow + 16 * w + 4 * k) = h;
+    }
+  }
+  __syncthreads();
+  grad_dense(w2, params + kOffB2, sH1, sH2, w, i, k);
+  __syncthreads();
+  grad_dense(w3, params + kOffB3, sH2, sH3, w, i, k);
+  __syncthreads();
+  // layer 4 on this wave's 16 samples, then the output gradient (lanes k == 0 hold outputs 0..3)
+  {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (k == 0) acc = *(const f32x4*)(params + kOffB4);
+    const f32x4* row = (const f32x4*)(sH3 + (16 * w + i) * kGradRow + 16 * k);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const f32x4 r = row[v];
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc = mfma_f32(w4[4 * v + e], r[e], acc);
+    }
+    const long sg = s0 + 16 * w + i;
+    if (k == 0) {
+      const bool valid = sg < nsamples;
+      const float inv_b = 1.0f / (float)batch;
+      float e = 0.0f;
+      f32x4 go;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const float d = acc[c] - (valid ? Y[sg * 4 + c] : 0.0f);
+        e = fmaxf(e, fabsf(d));
+        const float hg = fabsf(d) < kHuberBeta ? d / kHuberBeta : (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f));
+        go[c] = valid ? hg * (inv_b / 3.0f) : 0.0f;
+      }
+      go[3] = valid ? kErrWeight * (acc[3] - e) * inv_b : 0.0f;
+      *(f32x4*)(sGo + (16 * w + i) * 16) = go;
+    }
+  }
+  __syncthreads();
+
+  // backward: W4, b4 from the output gradient and h3
+  {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const int s = 16 * k + kk;
+      acc = mfma_f32(sH3[s * kGradRow + 16 * w + i], sGo[s * 16 + i], acc);
+    }
+    // D[4k + q][i] = dW4[i][16w + 4k + q]
+    if (i < kNnOut) *(f32x4*)(P + kOffW4 + i * kNnHid + 16 * w + 4 * k) = acc;
+    if (tid < kNnOut) {
+      float sb = 0.0f;
+      for (int s = 0; s < 64; s++) sb += sGo[s * 16 + tid];
+      P[kOffB4 + tid] = sb;
+    }
+  }
+  __syncthreads();
+  // delta3 = (W4[0:3]^T go) * (h3 > 0), over h3 (K = 4: output c = k)
+#pragma unroll
+  for (int sb = 0; sb < 4; sb++) {
+    const f32x4 acc = mfma_f32(w4t[0], sGo[(16 * sb + i) * 16 + k], (f32x4){0.0f, 0.0f, 0.0f, 0.0f});
+    f32x4* hp = (f32x4*)(sH3 + (16 * sb + i) * kGradRow + 16 * w + 4 * k);
+    const f32x4 h = *hp;
+    f32x4 d;
+#pragma unroll
+    for (int q = 0; q < 4; q++) d[q] = h[q] > 0.0f ? acc[q] : 0.0f;
+    *hp = d;
+  }
+  __syncthreads();
+  grad_weights(sH2, kGradRow, sH3, P + kOffW3, kNnHid, w, i, k);
+  if (w == 0) grad_bias(sH3, P + kOffB3, lane);
+  __syncthreads();
+  grad_delta(w3t, sH3, sH2, w, i, k);
+  __syncthreads();
+  grad_weights(sH1, kGradRow, sH2, P + kOffW2, kNnHid, w, i, k);
+  if (w == 0) grad_bias(sH2, P + kOffB2, lane);
+  __syncthreads();
+  grad_delta(w2t, sH2, sH1, w, i, k);
+  __syncthreads();
+  // W1 (input block m = 0: features 0..15, the first 12 kept), b1
+  if (w == 0) grad_bias(sH1, P + kOffB1, lane);
+  {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const int s = 16 * k + kk;
+      acc = mfma_f32(sX[s * 16 + i], sH1[s * kGradRow + 16 * w + i], acc);
+    }
+    // D[4k + q][i] = dW1[16w + i][4k + q]
+    if (4 * k < kNnIn) *(f32x4*)(P + kOffW1 + (16 * w + i) * kNnIn + 4 * k) = acc;
+  }
+}
+
+// g = the sum of the `nrows` rows of partial; Adam with bias corrections
+// c1 = 1/(1 - b1^t), c2 = 1/(1 - b2^t) supplied by the host.  A workgroup takes
+// 64 parameters: its four waves sum consecutive quarters of the rows, each in
+// row order, and wave 0 adds the four sums in order (deterministic).
 __global__ void __launch_bounds__(256)
 k_mlp_adam(float* __restrict__ params, float* __restrict__ m1, float* __restrict__ m2,
            const float* __restrict__ partial, int nrows, float lr, float b1, float b2, float eps, float c1,
            float c2, float* __restrict__ grad_out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kNnParams) return;
+  __shared__ float s_sum[4][64];
+  const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + lane;
+  const int per = (nrows + 3) / 4, r0 = wq * per, r1 = min(nrows, r0 + per);
   float g = 0.0f;
-  for (int r = 0; r < nrows; r++) g += partial[(size_t)r * kNnParams + p];
+  if (p < kNnParams)
+    for (int r = r0; r < r1; r++) g += partial[(size_t)r * kNnParams + p];
+  s_sum[wq][lane] = g;
+  __syncthreads();
+  if (wq != 0 || p >= kNnParams) return;
+  g = ((s_sum[0][lane] + s_sum[1][lane]) + s_sum[2][lane]) + s_sum[3][lane];
   if (grad_out) grad_out[p] = g;
   if (lr == 0.0f) return;  // gradient only
   const float m = b1 * m1[p] + (1.0f - b1) * g;
@@ -329,7 +421,6 @@ k_mlp_adam(float* __restrict__ params, float* __restrict__ m1, float* __restrict
 // 32kb + 16 + 4g + (e - 4) (e >= 4), and the weights are packed in that K order.
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // neuron feeding K slot (g, e) of K block kb in layers 2-4
 __device__ __forceinline__ int nn_kperm(int kb, int g, int e) {
