@@ -26,7 +26,7 @@ namespace tina {
 constexpr int kNnIn = 12, kNnHid = 64, kNnOut = 4;
 constexpr int kOffW1 = 0, kOffB1 = 768, kOffW2 = 832, kOffB2 = 4928, kOffW3 = 4992, kOffB3 = 9088;
 constexpr int kOffW4 = 9152, kOffB4 = 9408, kNnParams = 9412;
-constexpr int kNnBatchPerBlock = 64;      // samples per k_mlp_grad workgroup (four waves)
+constexpr int kNnBatchPerBlock = 64;      // samples per k_mlp_grad workgroup (eight waves)
 constexpr float kHuberBeta = 0.01f;
 constexpr float kErrWeight = 1.0f;
 
@@ -152,7 +152,7 @@ k_nn_analytic_rows(const ShadeArgs a_by_value, const float* __restrict__ X, long
 
 // ---- training ------------------------------------------------------------------
 //
-// One workgroup (four waves) per 64 samples; every product of the forward and
+// One workgroup (eight waves) per 64 samples; every product of the forward and
 // backward pass is a 16 x 16 tile of v_mfma_f32_16x16x4_f32 (float32 operands
 // and accumulation), lane l holding A[l % 16][l / 16], B[l / 16][l % 16] and
 // D[4 (l / 16) + q][l % 16], q = 0..3.  Activations live in LDS as one row per
@@ -161,8 +161,8 @@ k_nn_analytic_rows(const ShadeArgs a_by_value, const float* __restrict__ X, long
 // longer needs.  The K order inside a product is free as long as both operands
 // use it: layers 2-4 take input neuron c = 16 (l / 16) + kk at step kk, so a
 // lane reads its B values as four 16-byte vectors, and the weight gradients sum
-// over samples s = 16 (l / 16) + kk.  Each wave keeps the weights it multiplies
-// in registers, loaded once before the first product.  The workgroup's gradient
+// over samples in the order of grad_sample.  Each wave keeps the weights and
+// biases it uses in registers, loaded once before the first product.  The workgroup's gradient
 // goes to row blockIdx.x of `partial`.
 //
 // Loss per sample: mean_c smoothL1(o_c - y_c) + kErrWeight/2 (o_3 - max_c|o_c - y_c|)^2,
@@ -176,228 +176,305 @@ __device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Hout[s][16m + n] = relu(b[16m + n] + sum_c W[16m + n][c] Hin[s][c]) for the 64
-// samples (four 16-sample tiles), K = 64 in the order c = 16 k + kk.
-__device__ __forceinline__ void grad_dense(const float (&w)[16], const float* __restrict__ bias,
-                                           const float* Hin, float* Hout, int m, int i, int k) {
-  const f32x4 b0 = *(const f32x4*)(bias + 16 * m + 4 * k);
+// Hout[s][16m + n] = relu(b0 + sum_c W[16m + n][c] Hin[s][c]) for the 32 samples
+// of tiles sb0, sb0 + 1, K = 64 in the order c = 16 k + kk.
+__device__ __forceinline__ void grad_dense(const float (&w)[16], const f32x4 b0, const float* Hin, float* Hout, int m,
+                                           int sb0, int i, int k) {
+  f32x4 acc[2] = {b0, b0};
 #pragma unroll
-  for (int sb = 0; sb < 4; sb++) {
-    const f32x4* row = (const f32x4*)(Hin + (16 * sb + i) * kGradRow + 16 * k);
-    f32x4 acc = b0;
+  for (int v = 0; v < 4; v++) {
+    f32x4 r[2];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const f32x4 r = row[v];
+    for (int t = 0; t < 2; t++) r[t] = *(const f32x4*)(Hin + (16 * (sb0 + t) + i) * kGradRow + 16 * k + 4 * v);
 #pragma unroll
-      for (int e = 0; e < 4; e++) acc = mfma_f32(w[4 * v + e], r[e], acc);
-    }
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) acc[t] = mfma_f32(w[4 * v + e], r[t][e], acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
     f32x4 h;
 #pragma unroll
-    for (int q = 0; q < 4; q++) h[q] = fmaxf(acc[q], 0.0f);
-    *(f32x4*)(Hout + (16 * sb + i) * kGradRow + 16 * m + 4 * k) = h;
+    for (int q = 0; q < 4; q++) h[q] = fmaxf(acc[t][q], 0.0f);
+    *(f32x4*)(Hout + (16 * (sb0 + t) + i) * kGradRow + 16 * m + 4 * k) = h;
   }
 }
 
 // D[s][16m + n] = (H[s][16m + n] > 0) * sum_c W[c][16m + n] Dn[s][c] (A = W^T from
-// registers), written over H.
-__device__ __forceinline__ void grad_delta(const float (&wt)[16], const float* Dn, float* H, int m, int i, int k) {
+// registers) for the samples of tiles sb0, sb0 + 1, written over H.
+__device__ __forceinline__ void grad_delta(const float (&wt)[16], const float* Dn, float* H, int m, int sb0, int i,
+                                           int k) {
+  f32x4 acc[2];
 #pragma unroll
-  for (int sb = 0; sb < 4; sb++) {
-    const f32x4* row = (const f32x4*)(Dn + (16 * sb + i) * kGradRow + 16 * k);
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = 0; t < 2; t++) acc[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const f32x4 r = row[v];
+  for (int v = 0; v < 4; v++) {
+    f32x4 r[2];
 #pragma unroll
-      for (int e = 0; e < 4; e++) acc = mfma_f32(wt[4 * v + e], r[e], acc);
-    }
-    f32x4* hp = (f32x4*)(H + (16 * sb + i) * kGradRow + 16 * m + 4 * k);
+    for (int t = 0; t < 2; t++) r[t] = *(const f32x4*)(Dn + (16 * (sb0 + t) + i) * kGradRow + 16 * k + 4 * v);
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) acc[t] = mfma_f32(wt[4 * v + e], r[t][e], acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    f32x4* hp = (f32x4*)(H + (16 * (sb0 + t) + i) * kGradRow + 16 * m + 4 * k);
     const f32x4 h = *hp;
     f32x4 d;
 #pragma unroll
-    for (int q = 0; q < 4; q++) d[q] = h[q] > 0.0f ? acc[q] : 0.0f;
+    for (int q = 0; q < 4; q++) d[q] = h[q] > 0.0f ? acc[t][q] : 0.0f;
     *hp = d;
   }
 }
 
-// dW[16n + j][16m + r] = sum_s D[s][16n + j] H[s][16m + r] for this wave's input
-// block m and the four output blocks n, written transposed from the D layout
-// (float4 per lane) to the weight's rows of `out` (row length kin), then the
-// bias gradient sum_s D[s][o] by lanes (wave 0 only: o = lane).
+// Sample of step kk for lanes l / 16 = k in the weight-gradient sums: the
+// rotation by 4k puts the four k-groups' rows on disjoint LDS banks.
+__device__ __forceinline__ int grad_sample(int kk, int k) { return 16 * k + ((kk + 4 * k) & 15); }
+
+// dW[16n + j][16m + r] = sum_s D[s][16n + j] H[s][16m + r] for input block m and
+// output blocks n0, n0 + 1, written transposed from the D layout (float4 per
+// lane) to the weight's rows of `outW` (row length kin).
 __device__ __forceinline__ void grad_weights(const float* H, int hstride, const float* D, float* __restrict__ outW,
-                                             int kin, int m, int i, int k) {
+                                             int kin, int m, int n0, int i, int k) {
+  f32x4 acc[2];
 #pragma unroll
-  for (int n = 0; n < 4; n++) {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = 0; t < 2; t++) acc[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int kk = 0; kk < 16; kk++) {
-      const int s = 16 * k + kk;
-      acc = mfma_f32(H[s * hstride + 16 * m + i], D[s * kGradRow + 16 * n + i], acc);
-    }
-    const int r = 16 * m + 4 * k;  // input neurons r..r+3 of output neuron 16n + i
-    if (r < kin) *(f32x4*)(outW + (16 * n + i) * kin + r) = acc;
+  for (int kk = 0; kk < 16; kk++) {
+    const int s = grad_sample(kk, k);
+    const float a = H[s * hstride + 16 * m + i];
+#pragma unroll
+    for (int t = 0; t < 2; t++) acc[t] = mfma_f32(a, D[s * kGradRow + 16 * (n0 + t) + i], acc[t]);
   }
+  const int r = 16 * m + 4 * k;  // input neurons r..r+3 of output neuron 16n + i
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+    if (r < kin) *(f32x4*)(outW + (16 * (n0 + t) + i) * kin + r) = acc[t];
 }
 
-__device__ __forceinline__ void grad_bias(const float* D, float* __restrict__ outB, int o) {
-  float sb = 0.0f;
-  for (int s = 0; s < 64; s++) sb += D[s * kGradRow + o];
-  outB[o] = sb;
+// sum over the 16 sample steps of one tile as four interleaved chains (steps
+// kk = 4t + c into chain c), added in chain order
+__device__ __forceinline__ f32x4 chains4(const f32x4 (&a)[4]) { return ((a[0] + a[1]) + a[2]) + a[3]; }
+
+// the bias gradient sum_s D[s][o] of outputs 8w..8w+7: lane l sums samples
+// (l >> 3) + 8t, t = 0..7, of output 8w + (l & 7) in order, then the eight lane
+// groups are added by a fixed butterfly (deterministic)
+__device__ __forceinline__ void grad_bias(const float* D, float* __restrict__ outB, int w, int lane) {
+  const int o = 8 * w + (lane & 7), s0 = lane >> 3;
+  float v[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++) v[t] = D[(s0 + 8 * t) * kGradRow + o];
+  float sb = v[0];
+#pragma unroll
+  for (int t = 1; t < 8; t++) sb += v[t];
+  sb += __shfl_xor(sb, 8);
+  sb += __shfl_xor(sb, 16);
+  sb += __shfl_xor(sb, 32);
+  if (lane < 8) outB[o] = sb;
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))  // MFMA results in VGPRs
+constexpr int kGradBlock = 512;  // eight waves: two per SIMD, each with half of a layer's tiles
+static_assert(64 * 16 == 2 * kGradBlock, "the feature and gradient staging takes two passes");
+
+__global__ void __launch_bounds__(kGradBlock) __attribute__((amdgpu_waves_per_eu(2)))  // MFMA results in VGPRs
 k_mlp_grad(const float* __restrict__ params, const float* __restrict__ X, const float* __restrict__ Y,
            long first, long nsamples, int batch, float* __restrict__ partial) {
   __shared__ __attribute__((aligned(16))) float sX[64 * 16];   // [s][16]: features, 12..15 zero
   __shared__ __attribute__((aligned(16))) float sH1[64 * kGradRow], sH2[64 * kGradRow], sH3[64 * kGradRow];
   __shared__ __attribute__((aligned(16))) float sGo[64 * 16];  // [s][16]: output gradients, 4..15 zero
+  __shared__ __attribute__((aligned(16))) float sP[kNnParams];   // the parameters, staged once
   const int tid = threadIdx.x, lane = tid & 63, i = lane & 15, k = lane >> 4;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m = w & 3, h2 = 2 * (w >> 2);  // this wave's neuron block and first tile of the other dimension
   const long s0 = first + (long)blockIdx.x * 64;
   float* __restrict__ P = partial + (size_t)blockIdx.x * kNnParams;
 
-  // the weights this wave multiplies, in the K order of each product
-  float w1[3], w2[16], w3[16], w4[16], w4t[1], w3t[16], w2t[16];
+  // parameters, features and targets in with coalesced loads, all issued before the first store
+  constexpr int kPv = (kNnParams / 4 + kGradBlock - 1) / kGradBlock;  // float4 loads per thread
+  f32x4 pv[kPv];
 #pragma unroll
-  for (int kk = 0; kk < 3; kk++) w1[kk] = params[kOffW1 + (16 * w + i) * kNnIn + 4 * kk + k];
+  for (int t = 0; t < kPv; t++) {
+    const int e = tid + t * kGradBlock;
+    if (e < kNnParams / 4) pv[t] = ((const f32x4*)params)[e];
+  }
+  float xv[2];
 #pragma unroll
-  for (int kk = 0; kk < 16; kk++) {
-    w2[kk] = params[kOffW2 + (16 * w + i) * kNnHid + 16 * k + kk];
-    w3[kk] = params[kOffW3 + (16 * w + i) * kNnHid + 16 * k + kk];
-    w4[kk] = i < kNnOut ? params[kOffW4 + i * kNnHid + 16 * k + kk] : 0.0f;
-    w3t[kk] = params[kOffW3 + (16 * k + kk) * kNnHid + 16 * w + i];
-    w2t[kk] = params[kOffW2 + (16 * k + kk) * kNnHid + 16 * w + i];
+  for (int t = 0; t < 2; t++) {
+    const int e = tid + t * kGradBlock, s = e >> 4, f = e & 15;
+    xv[t] = (f < kNnIn && s0 + s < nsamples) ? X[(s0 + s) * kNnIn + f] : 0.0f;
   }
-  w4t[0] = k < 3 ? params[kOffW4 + k * kNnHid + 16 * w + i] : 0.0f;  // the colour rows of W4 only
-
-  for (int e = tid; e < 64 * 16; e += 256) {
-    const int s = e >> 4, f = e & 15;
-    sX[e] = (f < kNnIn && s0 + s < nsamples) ? X[(s0 + s) * kNnIn + f] : 0.0f;
-    sGo[e] = 0.0f;
+#pragma unroll
+  for (int t = 0; t < kPv; t++) {
+    const int e = tid + t * kGradBlock;
+    if (e < kNnParams / 4) ((f32x4*)sP)[e] = pv[t];
   }
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    sX[tid + t * kGradBlock] = xv[t];
+    sGo[tid + t * kGradBlock] = 0.0f;
+  }
+  // the targets of the 16 samples of waves 0-3 (their lanes k == 0 compute the output gradient)
+  const long sg = s0 + 16 * m + i;
+  const bool valid = sg < nsamples;
+  float y[3] = {0.0f, 0.0f, 0.0f};
+  if (w < 4 && k == 0 && valid)
+#pragma unroll
+    for (int c = 0; c < 3; c++) y[c] = Y[sg * 4 + c];
   __syncthreads();
 
+  // the weights and biases this wave uses, in the K order of each product
+  const float* params_l = sP;
+  float w1[3], w2[16], w3[16], w4[16], w4t[1], w3t[16], w2t[16];
+#pragma unroll
+  for (int kk = 0; kk < 3; kk++) w1[kk] = params_l[kOffW1 + (16 * m + i) * kNnIn + 4 * kk + k];
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++) {
+    w2[kk] = params_l[kOffW2 + (16 * m + i) * kNnHid + 16 * k + kk];
+    w3[kk] = params_l[kOffW3 + (16 * m + i) * kNnHid + 16 * k + kk];
+    w4[kk] = i < kNnOut ? params_l[kOffW4 + i * kNnHid + 16 * k + kk] : 0.0f;
+    w3t[kk] = params_l[kOffW3 + (16 * k + kk) * kNnHid + 16 * m + i];
+    w2t[kk] = params_l[kOffW2 + (16 * k + kk) * kNnHid + 16 * m + i];
+  }
+  w4t[0] = k < 3 ? params_l[kOffW4 + k * kNnHid + 16 * m + i] : 0.0f;  // the colour rows of W4 only
+  const f32x4 bias1 = *(const f32x4*)(params_l + kOffB1 + 16 * m + 4 * k);
+  const f32x4 bias2 = *(const f32x4*)(params_l + kOffB2 + 16 * m + 4 * k);
+  const f32x4 bias3 = *(const f32x4*)(params_l + kOffB3 + 16 * m + 4 * k);
+  const f32x4 bias4 = *(const f32x4*)(params_l + kOffB4);
   // forward: layer 1 (K = 12 in the order f = 4 kk + k), layers 2 and 3
   {
-    const f32x4 b0 = *(const f32x4*)(params + kOffB1 + 16 * w + 4 * k);
+    f32x4 a1[2] = {bias1, bias1};
 #pragma unroll
-    for (int sb = 0; sb < 4; sb++) {
-      f32x4 acc = b0;
+    for (int kk = 0; kk < 3; kk++)
 #pragma unroll
-      for (int kk = 0; kk < 3; kk++) acc = mfma_f32(w1[kk], sX[(16 * sb + i) * 16 + 4 * kk + k], acc);
-      f32x4 h;
+      for (int t = 0; t < 2; t++) a1[t] = mfma_f32(w1[kk], sX[(16 * (h2 + t) + i) * 16 + 4 * kk + k], a1[t]);
 #pragma unroll
-      for (int q = 0; q < 4; q++) h[q] = fmaxf(acc[q], 0.0f);
-      *(f32x4*)(sH1 + (16 * sb + i) * kGradRow + 16 * w + 4 * k) = h;
+    for (int t = 0; t < 2; t++) {
+      f32x4 hv;
+#pragma unroll
+      for (int q = 0; q < 4; q++) hv[q] = fmaxf(a1[t][q], 0.0f);
+      *(f32x4*)(sH1 + (16 * (h2 + t) + i) * kGradRow + 16 * m + 4 * k) = hv;
     }
   }
   __syncthreads();
-  grad_dense(w2, params + kOffB2, sH1, sH2, w, i, k);
+  grad_dense(w2, bias2, sH1, sH2, m, h2, i, k);
   __syncthreads();
-  grad_dense(w3, params + kOffB3, sH2, sH3, w, i, k);
+  grad_dense(w3, bias3, sH2, sH3, m, h2, i, k);
   __syncthreads();
-  // layer 4 on this wave's 16 samples, then the output gradient (lanes k == 0 hold outputs 0..3)
-  {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (k == 0) acc = *(const f32x4*)(params + kOffB4);
-    const f32x4* row = (const f32x4*)(sH3 + (16 * w + i) * kGradRow + 16 * k);
+  // layer 4 on the 16 samples of tile m (waves 0-3), then the output gradient
+  // (lanes k == 0 hold outputs 0..3)
+  if (w < 4) {
+    f32x4 ch[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) ch[c] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    if (k == 0) ch[0] = bias4;
+    const f32x4* row = (const f32x4*)(sH3 + (16 * m + i) * kGradRow + 16 * k);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const f32x4 r = row[v];
 #pragma unroll
-      for (int e = 0; e < 4; e++) acc = mfma_f32(w4[4 * v + e], r[e], acc);
+      for (int e = 0; e < 4; e++) ch[e] = mfma_f32(w4[4 * v + e], r[e], ch[e]);
     }
-    const long sg = s0 + 16 * w + i;
+    const f32x4 acc = chains4(ch);
     if (k == 0) {
-      const bool valid = sg < nsamples;
       const float inv_b = 1.0f / (float)batch;
       float e = 0.0f;
       f32x4 go;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const float d = acc[c] - (valid ? Y[sg * 4 + c] : 0.0f);
+        const float d = acc[c] - y[c];
         e = fmaxf(e, fabsf(d));
         const float hg = fabsf(d) < kHuberBeta ? d / kHuberBeta : (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f));
         go[c] = valid ? hg * (inv_b / 3.0f) : 0.0f;
       }
       go[3] = valid ? kErrWeight * (acc[3] - e) * inv_b : 0.0f;
-      *(f32x4*)(sGo + (16 * w + i) * 16) = go;
+      *(f32x4*)(sGo + (16 * m + i) * 16) = go;
     }
   }
   __syncthreads();
 
-  // backward: W4, b4 from the output gradient and h3
-  {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  // backward: W4, b4 from the output gradient and h3 (waves 0-3: input block m)
+  if (w < 4) {
+    f32x4 ch[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) ch[c] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
-      const int s = 16 * k + kk;
-      acc = mfma_f32(sH3[s * kGradRow + 16 * w + i], sGo[s * 16 + i], acc);
+      const int s = grad_sample(kk, k);
+      ch[kk & 3] = mfma_f32(sH3[s * kGradRow + 16 * m + i], sGo[s * 16 + i], ch[kk & 3]);
     }
-    // D[4k + q][i] = dW4[i][16w + 4k + q]
-    if (i < kNnOut) *(f32x4*)(P + kOffW4 + i * kNnHid + 16 * w + 4 * k) = acc;
-    if (tid < kNnOut) {
-      float sb = 0.0f;
-      for (int s = 0; s < 64; s++) sb += sGo[s * 16 + tid];
-      P[kOffB4 + tid] = sb;
-    }
+    const f32x4 acc = chains4(ch);
+    // D[4k + q][i] = dW4[i][16m + 4k + q]
+    if (i < kNnOut) *(f32x4*)(P + kOffW4 + i * kNnHid + 16 * m + 4 * k) = acc;
+  } else if (tid - 256 < kNnOut) {
+    float sb = 0.0f;
+    for (int s = 0; s < 64; s++) sb += sGo[s * 16 + (tid - 256)];
+    P[kOffB4 + tid - 256] = sb;
   }
   __syncthreads();
   // delta3 = (W4[0:3]^T go) * (h3 > 0), over h3 (K = 4: output c = k)
 #pragma unroll
-  for (int sb = 0; sb < 4; sb++) {
+  for (int t = 0; t < 2; t++) {
+    const int sb = h2 + t;
     const f32x4 acc = mfma_f32(w4t[0], sGo[(16 * sb + i) * 16 + k], (f32x4){0.0f, 0.0f, 0.0f, 0.0f});
-    f32x4* hp = (f32x4*)(sH3 + (16 * sb + i) * kGradRow + 16 * w + 4 * k);
-    const f32x4 h = *hp;
+    f32x4* hp = (f32x4*)(sH3 + (16 * sb + i) * kGradRow + 16 * m + 4 * k);
+    const f32x4 hv = *hp;
     f32x4 d;
 #pragma unroll
-    for (int q = 0; q < 4; q++) d[q] = h[q] > 0.0f ? acc[q] : 0.0f;
+    for (int q = 0; q < 4; q++) d[q] = hv[q] > 0.0f ? acc[q] : 0.0f;
     *hp = d;
   }
   __syncthreads();
-  grad_weights(sH2, kGradRow, sH3, P + kOffW3, kNnHid, w, i, k);
-  if (w == 0) grad_bias(sH3, P + kOffB3, lane);
+  grad_weights(sH2, kGradRow, sH3, P + kOffW3, kNnHid, m, h2, i, k);
+  grad_bias(sH3, P + kOffB3, w, lane);
   __syncthreads();
-  grad_delta(w3t, sH3, sH2, w, i, k);
+  grad_delta(w3t, sH3, sH2, m, h2, i, k);
   __syncthreads();
-  grad_weights(sH1, kGradRow, sH2, P + kOffW2, kNnHid, w, i, k);
-  if (w == 0) grad_bias(sH2, P + kOffB2, lane);
+  grad_weights(sH1, kGradRow, sH2, P + kOffW2, kNnHid, m, h2, i, k);
+  grad_bias(sH2, P + kOffB2, w, lane);
   __syncthreads();
-  grad_delta(w2t, sH2, sH1, w, i, k);
+  grad_delta(w2t, sH2, sH1, m, h2, i, k);
   __syncthreads();
-  // W1 (input block m = 0: features 0..15, the first 12 kept), b1
-  if (w == 0) grad_bias(sH1, P + kOffB1, lane);
-  {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  // W1 (input block 0: features 0..15, the first 12 kept; waves 0-3: output block m), b1
+  grad_bias(sH1, P + kOffB1, w, lane);
+  if (w < 4) {
+    f32x4 ch[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) ch[c] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
-      const int s = 16 * k + kk;
-      acc = mfma_f32(sX[s * 16 + i], sH1[s * kGradRow + 16 * w + i], acc);
+      const int s = grad_sample(kk, k);
+      ch[kk & 3] = mfma_f32(sX[s * 16 + i], sH1[s * kGradRow + 16 * m + i], ch[kk & 3]);
     }
-    // D[4k + q][i] = dW1[16w + i][4k + q]
-    if (4 * k < kNnIn) *(f32x4*)(P + kOffW1 + (16 * w + i) * kNnIn + 4 * k) = acc;
+    const f32x4 acc = chains4(ch);
+    // D[4k + q][i] = dW1[16m + i][4k + q]
+    if (4 * k < kNnIn) *(f32x4*)(P + kOffW1 + (16 * m + i) * kNnIn + 4 * k) = acc;
   }
 }
 
 // g = the sum of the `nrows` rows of partial; Adam with bias corrections
 // c1 = 1/(1 - b1^t), c2 = 1/(1 - b2^t) supplied by the host.  A workgroup takes
-// 64 parameters: its four waves sum consecutive quarters of the rows, each in
-// row order, and wave 0 adds the four sums in order (deterministic).
-__global__ void __launch_bounds__(256)
+// 64 parameters: its kAdamGroups waves sum consecutive runs of the rows, each in
+// row order, and wave 0 adds the runs' sums in order (deterministic).
+constexpr int kAdamGroups = 16;
+
+__global__ void __launch_bounds__(64 * kAdamGroups)
 k_mlp_adam(float* __restrict__ params, float* __restrict__ m1, float* __restrict__ m2,
            const float* __restrict__ partial, int nrows, float lr, float b1, float b2, float eps, float c1,
            float c2, float* __restrict__ grad_out) {
-  __shared__ float s_sum[4][64];
+  __shared__ float s_sum[kAdamGroups][64];
   const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
   const int p = blockIdx.x * 64 + lane;
-  const int per = (nrows + 3) / 4, r0 = wq * per, r1 = min(nrows, r0 + per);
+  const int per = (nrows + kAdamGroups - 1) / kAdamGroups, r0 = wq * per, r1 = min(nrows, r0 + per);
   float g = 0.0f;
   if (p < kNnParams)
     for (int r = r0; r < r1; r++) g += partial[(size_t)r * kNnParams + p];
   s_sum[wq][lane] = g;
   __syncthreads();
   if (wq != 0 || p >= kNnParams) return;
-  g = ((s_sum[0][lane] + s_sum[1][lane]) + s_sum[2][lane]) + s_sum[3][lane];
+  g = s_sum[0][lane];
+#pragma unroll
+  for (int q = 1; q < kAdamGroups; q++) g += s_sum[q][lane];
   if (grad_out) grad_out[p] = g;
   if (lr == 0.0f) return;  // gradient only
   const float m = b1 * m1[p] + (1.0f - b1) * g;

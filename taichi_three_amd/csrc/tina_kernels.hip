@@ -811,13 +811,13 @@ int tina_mlp_train_step(const float* d_x, const float* d_y, int64_t first, int64
     return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_train_step: the batch must lie inside the samples");
   if (step < 1 || !(lr >= 0.0f)) return fail(nullptr, TINA_ERR_INVALID, "tina_mlp_train_step: step >= 1, lr >= 0");
   const int nblocks = (batch + kNnBatchPerBlock - 1) / kNnBatchPerBlock;
-  hipLaunchKernelGGL(k_mlp_grad, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, d_params, d_x, d_y, (long)first,
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nblocks), dim3(kGradBlock), 0, (hipStream_t)stream, d_params, d_x, d_y, (long)first,
                      (long)(first + batch), batch, d_partial);
   int rc = hip_check(nullptr, hipGetLastError(), "k_mlp_grad");
   if (rc) return rc;
   const double b1 = 0.9, b2 = 0.999;
   const float c1 = (float)(1.0 / (1.0 - pow(b1, step))), c2 = (float)(1.0 / (1.0 - pow(b2, step)));
-  hipLaunchKernelGGL(k_mlp_adam, dim3((kNnParams + 63) / 64), dim3(256), 0, (hipStream_t)stream, d_params, d_m1,
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kNnParams + 63) / 64), dim3(64 * kAdamGroups), 0, (hipStream_t)stream, d_params, d_m1,
                      d_m2, d_partial, nblocks, lr, (float)b1, (float)b2, 1e-8f, c1, c2, d_grad);
   return hip_check(nullptr, hipGetLastError(), "k_mlp_adam");
 }
