@@ -19,6 +19,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'fps & Mtri/s at 1080p/2M tris; neural vs analytic shade ms @ PSNR≥40dB'
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0                         # dense bf16 MFMA peak of one MI355X
+MLP_FLOP_PER_ROW = 2 * (12 * 64 + 64 * 64 * 2 + 64 * 4)   # the 12-64-64-64-4 network, one row
 
 
 def cpu_info():
@@ -131,9 +133,27 @@ def eval_only(ns, n, reps):
     t_nn, t_an = timed(ns.infer), timed(ns.analytic)
     o_nn = ns.infer(X)[:, :3].cpu().numpy()
     o_an = ns.analytic(X)[:, :3].cpu().numpy()
+    flop = n * MLP_FLOP_PER_ROW
+    # the floor of one launch in this timing (back-to-back calls of a one-element fill)
+    z = torch.zeros(1, device='cuda')
+    t_launch = timed(lambda _x: z.fill_(1.0))
+    # four times the rows: the per-row rates once the fixed cost of a launch is amortised
+    X4 = torch.cat([X] * 4)
+    t_nn4 = timed(lambda _x: ns.infer(X4))
+    t_an4 = timed(lambda _x: ns.analytic(X4))
+    del X4
     return {'rows': n, 'analytic_ms': round(t_an, 4), 'mlp_ms': round(t_nn, 4),
             'speedup': round(t_an / t_nn, 3), 'psnr_db': round(psnr(o_nn, o_an), 2),
-            'mlp_tflops': round(n * 2 * (12 * 64 + 64 * 64 * 2 + 64 * 4) / (t_nn * 1e-3) / 1e12, 2)}
+            'mlp_tflops': round(flop / (t_nn * 1e-3) / 1e12, 2),
+            'roofline': {'bound': 'mfma', 'kernel': 'k_mlp_rows', 'achieved': round(flop / (t_nn * 1e-3) / 1e12, 1),
+                         'peak': BF16_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                         'frac': round(flop / (t_nn * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4), 'traffic': None,
+                         'flop_per_row': MLP_FLOP_PER_ROW,
+                         'mfma_floor_ms': round(flop / (BF16_PEAK_TFLOPS * 1e12) * 1e3, 4),
+                         'launch_floor_ms': round(t_launch, 4)},
+            'rows_x4': {'rows': 4 * n, 'analytic_ms': round(t_an4, 4), 'mlp_ms': round(t_nn4, 4),
+                        'speedup': round(t_an4 / t_nn4, 3),
+                        'mlp_tflops': round(4 * flop / (t_nn4 * 1e-3) / 1e12, 1)}}
 
 
 def neural_vs_analytic(reps, train_steps):
@@ -162,6 +182,17 @@ def neural_vs_analytic(reps, train_steps):
     neural = stage_times(scene, reps)['shade']
     ids = scene.engine.ids.to_numpy()
     covered = int((ids >= 0).sum())
+    # the same trained network at other thresholds: fewer analytic pixels, lower PSNR
+    # (inf: no fallback at all)
+    sweep = {}
+    for th in (0.03, float('inf')):
+        ns.threshold = th
+        scene.render()
+        torch.cuda.synchronize()
+        sweep[str(th)] = {'fallback_frac': round(ns.fallback_count(scene.engine) / max(1, covered), 4),
+                          'psnr_db': round(psnr(scene.img.to_numpy(), ref), 2),
+                          'neural_shade_ms': round(stage_times(scene, reps)['shade'], 4)}
+    ns.threshold = 1e-2
     rows = eval_only(ns, covered, reps)
     return {'workload': 'ibl_1080p: 1920x1080, 1M-face soup, PBR under a 512x256 procedural HDR sky + 1 sun',
             'analytic_shade_ms': round(analytic, 4), 'neural_shade_ms': round(neural, 4),
@@ -169,8 +200,9 @@ def neural_vs_analytic(reps, train_steps):
             'eval_only': rows,
             'fallback_pixels': nfall, 'covered_pixels': covered,
             'fallback_frac': round(nfall / max(1, covered), 4), 'threshold': ns.threshold,
+            'other_thresholds': sweep,
             'train': {'samples': 1 << 24, 'steps': train_steps, 'batch': 16384, 'seconds': round(t_train, 2)},
-            'mlp_flop_per_px': 2 * (12 * 64 + 64 * 64 * 2 + 64 * 4)}
+            'mlp_flop_per_px': MLP_FLOP_PER_ROW}
 
 
 def post_passes(reps):

@@ -277,7 +277,7 @@ __device__ __forceinline__ void grad_bias(const float* D, float* __restrict__ ou
 constexpr int kGradBlock = 512;  // eight waves: two per SIMD, each with half of a layer's tiles
 static_assert(64 * 16 == 2 * kGradBlock, "the feature and gradient staging takes two passes");
 
-__global__ void __launch_bounds__(kGradBlock) __attribute__((amdgpu_waves_per_eu(2)))  // MFMA results in VGPRs
+__global__ void __launch_bounds__(kGradBlock) __attribute__((amdgpu_waves_per_eu(2)))
 k_mlp_grad(const float* __restrict__ params, const float* __restrict__ X, const float* __restrict__ Y,
            long first, long nsamples, int batch, float* __restrict__ partial) {
   __shared__ __attribute__((aligned(16))) float sX[64 * 16];   // [s][16]: features, 12..15 zero

@@ -181,9 +181,8 @@ __device__ __forceinline__ void nn_rows_load(const float* __restrict__ X, long n
 // the current ones; the loads and stores are raw buffer operations whose
 // out-of-range lanes read zeros and write nothing, so the loop has no branch
 // around them and the wait before the next rows leaves the stores in flight.
-// Three waves per SIMD: with that bound the MFMAs accumulate in VGPRs (under a
-// one-wave bound the compiler keeps their results in AGPRs and copies every
-// value out with v_accvgpr_read before the activation).
+// Three waves per SIMD: the register budget that keeps three 64-row passes in
+// flight on each SIMD.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_mlp_rows(const __bf16* __restrict__ pk, const float* __restrict__ X, long n, float* __restrict__ O) {
   __shared__ __attribute__((aligned(16))) float s_b[3 * 64 + 16];
