@@ -8,7 +8,7 @@
 //            reserves the workgroup's block of bin entries; the workgroup writes,
 //            per tile, its (offset, count) chunk record and scatters its face ids
 //            into its block.  Faces whose bbox touches more than kBinMaxTiles
-//            tiles go to the giant queue (k_giant, global atomics) instead.
+//            tiles go to the giant queue instead.
 //   k_tile   one workgroup per screen tile.  The tile's chunks from all binning
 //            workgroups form one virtual list (prefix sum of their counts in LDS).
 //            The tile's keys are loaded into LDS once (or start empty when the
@@ -18,9 +18,9 @@
 //            expansion (the bbox pixels of the wave's faces laid end to end, one
 //            pixel per lane per step), reducing every covered pixel with an LDS
 //            64-bit atomicMin.  Waves run independently (no workgroup barrier in
-//            the loop).  The keys go back to HBM once, coalesced.
-//   k_giant  after k_tile: the faces whose bbox spans more than max_tiles tiles,
-//            with global 64-bit atomics.
+//            the loop).  The giant faces follow the binned ones in every tile's
+//            list (each tile takes the part of each giant inside it).  The keys
+//            go back to HBM once, coalesced.
 //
 // No global atomic touches the key buffer on this path: the 64-bit atomic rate
 // to HBM-resident lines (about 27 G/s measured on MI355X) would bound the pass.
@@ -54,8 +54,9 @@ inline int bin_faces_for(long ntiles) {
   return 0;
 }
 
-// cursor: entries reserved by the binning workgroups; giant: faces queued for
-// k_giant; done: k_giant workgroups finished (the last one resets all three).
+// cursor: entries reserved by the binning workgroups; giant: faces in the giant
+// queue.  A single raster call alternates between two of them, k_bin emptying
+// the one the next call uses; a batch empties its scenes' after the tile pass.
 struct BinCounters {
   uint32_t cursor, giant, done, pad;
 };
@@ -83,6 +84,7 @@ struct BinJob {
   FaceRec* recs;
   BigRec* giantq;
   BinCounters* ctr;
+  BinCounters* ctr_next;  // zeroed by k_bin for the next call (counters alternate), or null
   Cam cam;
   int T;
   uint32_t id_base, flags;
@@ -287,18 +289,6 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
 // Faces too large to bin: every lane of the grid takes a slice of each one and
 // reduces into the key buffer with global atomics.  Runs after k_tile has stored
 // its tiles (a minimum does not depend on the order it is taken in).
-__device__ __forceinline__ void giant_body(const BinJob& j, uint32_t ng) {
-  const long stride = (long)gridDim.x * kBlock;
-  for (uint32_t r = 0; r < ng; r++) {
-    const BigRec rec = j.giantq[r];
-    const long w = rec.s.topx - rec.s.botx + 1;
-    const long area = w * (long)(rec.s.topy - rec.s.boty + 1);
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < area; i += stride) {
-      const long dy = i / w;
-      cover_pixel(rec.s, j.cam, rec.s.botx + (int)(i - dy * w), rec.s.boty + (int)dy, rec.id, j.keys);
-    }
-  }
-}
 
 // Makes this lane's LDS writes visible to the rest of its wave before they are read.
 __device__ __forceinline__ void wave_sync_lds() {
@@ -330,9 +320,13 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   const int ntiles = j.ntiles;  // chunk tables are group-major: [group][tile]
   // (Staging the chunks' entry offsets in LDS with the counts, to take one
   // dependent read out of the face loop, measured no faster.)
-  const uint32_t total = block_exclusive_scan<kBlock>(
+  const uint32_t nbin = block_exclusive_scan<kBlock>(
       ngroups, [&](int c) { return chunk_cnt[(size_t)c * ntiles + t]; },
       [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);
+  // the giant faces (not binned: more than max_tiles tiles) follow the binned
+  // ones in the tile's list; each tile takes the part of every giant inside it
+  const uint32_t ngiant = __atomic_load_n(&j.ctr->giant, __ATOMIC_RELAXED);
+  const uint32_t total = nbin + ngiant;
   const int tw = min(kTile, cam.W - ox), th = min(kTile, cam.H - oy);
   if (total == 0) {
     if (!j.load_keys)  // the deferred clear of a tile no face touches
@@ -343,7 +337,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     return;
   }
   if (tid == 0) {
-    pre[ngroups] = total;
+    pre[ngroups] = nbin;
     s_next = 0;
   }
   for (int i = tid; i < kTile * kTile; i += kBlock) {
@@ -360,7 +354,23 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     const uint32_t v = b0 + lane;
     int area = 0;
     float4 ra, rb, rc, rd;
-    if (v < total) {
+    if (v >= nbin && v < total) {
+      const BigRec& g = j.giantq[v - nbin];
+      const FaceSetup& gs = g.s;
+      const int bx0 = max(gs.botx, ox), by0 = max(gs.boty, oy);
+      const int bx1 = min(gs.topx, ox + tw - 1), by1 = min(gs.topy, oy + th - 1);
+      const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
+      if (w > 0 && h > 0) {
+        area = w * h;
+        ra = make_float4(gs.bx, gs.by, gs.cx, gs.cy);
+        rb = make_float4(gs.bcnx, gs.bcny, gs.canx, gs.cany);
+        rc = make_float4(gs.ws0, gs.ws1, gs.ws2, gs.Az);
+        rd = make_float4(gs.Bz, gs.Cz,
+                         __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) |
+                                        ((face_well_conditioned(gs) ? 1 : 0) << 24)),
+                         __int_as_float((int)g.id));
+      }
+    } else if (v < total) {
       int lo = 0, hi = ngroups;  // pre[lo] <= v < pre[hi]
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -498,31 +508,10 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
 
 __global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
+  // the next call's counters were this call's predecessor's: every kernel of
+  // that call has finished (stream order), so one thread can empty them
+  if (j.ctr_next && blockIdx.x == 0 && threadIdx.x == 0) *j.ctr_next = BinCounters{0u, 0u, 0u, 0u};
   bin_group(j, blockIdx.x);
-}
-
-// The last workgroup of the giant pass to finish empties the counters for the
-// next raster call; every workgroup has read the giant count before it takes its
-// ticket.
-__device__ __forceinline__ void giant_ticket(BinCounters* ctr) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t t = atomicAdd(&ctr->done, 1u);
-    if (t == gridDim.x - 1) {
-      __atomic_store_n(&ctr->cursor, 0u, __ATOMIC_RELAXED);
-      __atomic_store_n(&ctr->giant, 0u, __ATOMIC_RELAXED);
-      __atomic_store_n(&ctr->done, 0u, __ATOMIC_RELAXED);
-    }
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_giant(const BinJob jv) {
-  TINA_KERNARG_JOB(BinJob, j, jv);
-  __shared__ uint32_t s_ng;
-  if (threadIdx.x == 0) s_ng = __atomic_load_n(&j.ctr->giant, __ATOMIC_RELAXED);
-  __syncthreads();
-  giant_body(j, s_ng);
-  giant_ticket(j.ctr);
 }
 
 // Workgroup b of n -> tile.  The dispatcher deals workgroups round-robin over
@@ -572,21 +561,10 @@ k_bbin(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njob
 // scenes) rather than one dependent load per scene.  Runs after k_btile; the
 // last workgroup to finish empties every scene's counters (scene 0's `done`
 // counts the workgroups).
-__global__ void __launch_bounds__(kBlock) k_bgiant(const BinJob* __restrict__ jobs, int njobs) {
-  __shared__ uint32_t s_ng[kBlock];
-  if (threadIdx.x < njobs) s_ng[threadIdx.x] = __atomic_load_n(&jobs[threadIdx.x].ctr->giant, __ATOMIC_RELAXED);
-  __syncthreads();
-  for (int s = 0; s < njobs; s++)
-    if (s_ng[s]) giant_body(jobs[s], s_ng[s]);
-  __syncthreads();
-  __shared__ uint32_t s_last;
-  if (threadIdx.x == 0) s_last = atomicAdd(&jobs[0].ctr->done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (s_last && threadIdx.x < njobs) {
-    __atomic_store_n(&jobs[threadIdx.x].ctr->cursor, 0u, __ATOMIC_RELAXED);
-    __atomic_store_n(&jobs[threadIdx.x].ctr->giant, 0u, __ATOMIC_RELAXED);
-    __atomic_store_n(&jobs[threadIdx.x].ctr->done, 0u, __ATOMIC_RELAXED);
-  }
+// Empties every scene's counters for the next launch of the batch (after its
+// tile pass: nothing reads them any more).
+__global__ void __launch_bounds__(kBlock) k_breset(const BinJob* __restrict__ jobs, int njobs) {
+  for (int s = threadIdx.x; s < njobs; s += kBlock) *jobs[s].ctr = BinCounters{0u, 0u, 0u, 0u};
 }
 
 __global__ void __launch_bounds__(kBlock)

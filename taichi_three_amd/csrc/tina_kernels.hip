@@ -40,7 +40,8 @@ struct tina_ctx {
   uint32_t* chunk_cnt = nullptr;
   uint32_t* entries = nullptr;
   FaceRec* recs = nullptr;
-  BinCounters* binctr = nullptr;
+  BinCounters* binctr = nullptr;  // two: a raster call uses one and empties the other
+  int bin_parity = 0;
   // learned-shading fallback queue (kFbSeg segments of fb_segcap pixel indices),
   // two buffers of segment counters, and k_shade_nn's resident grid
   uint32_t* fbq = nullptr;
@@ -343,7 +344,7 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
   const long nchunks = ((long)width * height + kNnShadePix - 1) / kNnShadePix;
   c->fb_segcap = (nchunks + kFbSeg - 1) / kFbSeg * kNnShadePix;
   if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
-      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, sizeof(BinCounters)) != hipSuccess ||
+      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, 2 * sizeof(BinCounters)) != hipSuccess ||
       hipMalloc(&c->fbq, sizeof(uint32_t) * (size_t)c->fb_segcap * kFbSeg) != hipSuccess ||
       hipMalloc(&c->fbn, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess) {
     (void)hipGetLastError();
@@ -351,7 +352,7 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
     return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the depth|id buffer failed");
   }
   if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess ||
-      hipMemset(c->binctr, 0, sizeof(BinCounters)) != hipSuccess ||
+      hipMemset(c->binctr, 0, 2 * sizeof(BinCounters)) != hipSuccess ||
       hipMemset(c->fbn, 0, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     release();
@@ -427,7 +428,9 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.entries = c->entries;
     j.recs = c->recs;
     j.giantq = c->giantq;
-    j.ctr = c->binctr;
+    j.ctr = c->binctr + c->bin_parity;
+    j.ctr_next = c->binctr + (c->bin_parity ^ 1);
+    c->bin_parity ^= 1;
     j.cam = cam;
     j.T = nfaces;
     j.id_base = id_base;
@@ -446,9 +449,6 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     TINA_LAUNCH(c, "k_bin");
     hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), (size_t)(j.ngroups + 1) * 4, s, j);
     TINA_LAUNCH(c, "k_tile");
-    // usually no face is giant: a quarter of the queue grid dispatches in a quarter of the time
-    hipLaunchKernelGGL(k_giant, dim3(kQueueGrid / 4), dim3(kBlock), 0, s, j);
-    TINA_LAUNCH(c, "k_giant");
     return TINA_OK;
   }
   // resolutions or face counts beyond the binned path's LDS tables: global atomics
@@ -1101,6 +1101,7 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     j.recs = (FaceRec*)rc_;
     j.giantq = (BigRec*)gq;
     j.ctr = (BinCounters*)ct;
+    j.ctr_next = nullptr;
     memcpy(j.cam.W2V, d.cam.W2V, sizeof(j.cam.W2V));
     memcpy(j.cam.V2W, d.cam.V2W, sizeof(j.cam.V2W));
     j.cam.bias[0] = d.cam.bias[0];
@@ -1180,8 +1181,8 @@ int tina_batch_render(tina_batch* b, void* stream) {
   hipLaunchKernelGGL(k_btile, dim3(batch_grid(b->tune.grid_tile, b->totals[1])), dim3(kBlock),
                      (size_t)(b->max_groups + 1) * 4, s, b->d_jobs, b->d_prefix + (n + 1), n);
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_btile"))) return rc;
-  hipLaunchKernelGGL(k_bgiant, dim3(kQueueGrid), dim3(kBlock), 0, s, b->d_jobs, n);
-  if ((rc = hip_check(nullptr, hipGetLastError(), "k_bgiant"))) return rc;
+  hipLaunchKernelGGL(k_breset, dim3(1), dim3(kBlock), 0, s, b->d_jobs, n);
+  if ((rc = hip_check(nullptr, hipGetLastError(), "k_breset"))) return rc;
   const unsigned gs = batch_grid(b->tune.grid_shade, b->totals[2]);
   const int* pre = b->d_prefix + 2 * (n + 1);
   switch (b->prog) {

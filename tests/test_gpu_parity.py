@@ -257,6 +257,39 @@ def test_multi_object_ties_and_probe(gpu):
     assert np.array_equal(el, exp)
 
 
+def test_giant_faces_over_a_loaded_frame_repeatedly(gpu):
+    """A soup of small faces, then an object of giant faces (bboxes over many
+    tiles, some crossing the soup in depth) drawn into the same keys, three
+    frames in a row: the tiles take each giant's part inside them whether they
+    start from empty keys or load the first object's, and the counters the
+    raster calls alternate between stay right from frame to frame."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import workloads as wl
+    rng = np.random.default_rng(23)
+    small = wl.triangle_soup(rng, 20000, np.log(0.03), 0.5)
+    big = (rng.uniform(-1.5, 1.5, (40, 3, 3)) * np.array([1.0, 1.0, 0.6])).astype(np.float32)
+    W, H = 480, 360
+    scene = tina.Scene(W, H, maxfaces=len(small) + len(big))
+    view, proj = wl.default_camera(W / H)
+    scene.engine.set_camera(view, proj)
+    ms, mb = tina.SimpleMesh(), tina.SimpleMesh()
+    ms.set_face_verts(small)
+    mb.set_face_verts(big)
+    scene.add_object(ms)
+    scene.add_object(mb, tina.Classic(color=(0.3, 0.6, 0.9)))
+    cam = orc.camera(scene.engine.W2V, scene.engine.V2W)
+    od, oi = orc.raster(W, H, cam, small, id_base=0)
+    od, oi = orc.raster(W, H, cam, big, id_base=len(small), depth=od, occup=oi)
+    assert (oi >= len(small)).sum() > 20000 and ((oi >= 0) & (oi < len(small))).sum() > 5000
+    for frame in range(3):
+        print(f'case: giant faces over a loaded frame, frame {frame}', flush=True)
+        scene.render()
+        torch.cuda.synchronize()
+        assert np.array_equal(scene.engine.ids.to_numpy().T, oi)
+        assert np.array_equal(scene.engine.depth.to_numpy().T, od)
+
+
 def test_deterministic_repeat(gpu):
     import torch
     from taichi_three_amd import workloads as wl
