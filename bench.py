@@ -443,7 +443,7 @@ def main():
         'covered_pixels': covered,
         'frame_roofline': {'bytes': frame_bytes, 'achieved_GBs': round(frame_bytes / (ms * 1e-3) / 1e9, 1),
                            'frac': round(frame_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        'roofline': {'bound': 'hbm', 'kernel': {'clear': '(none)', 'raster': 'k_bin+k_tile+k_giant',
+        'roofline': {'bound': 'hbm', 'kernel': {'clear': '(none)', 'raster': 'k_bin+k_tile',
                                                 'shade': 'k_shade'}[dom],
                      'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic_for(dom)},
