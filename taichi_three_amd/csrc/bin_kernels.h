@@ -84,7 +84,8 @@ struct BinJob {
   FaceRec* recs;
   BigRec* giantq;
   BinCounters* ctr;
-  BinCounters* ctr_next;  // zeroed by k_bin for the next call (counters alternate), or null
+  BinCounters* ctr_next;  // zeroed for the next call (counters alternate), or null
+  uint32_t* spans;        // [T] per-face tile spans written by k_setup, or null: k_bin sets the faces up
   Cam cam;
   int T;
   uint32_t id_base, flags;
@@ -190,6 +191,43 @@ __device__ __forceinline__ bool tight_bbox(const Cam& cam, FaceSetup& s) {
 
 // Binning workgroup g of job j (the body of k_bin).  Needs (ntiles + bin_faces)
 // words of dynamic LDS.
+// Set-up of face f (cull, clip, bbox, edges; triangle.py:93-113) and its bbox
+// shrunk to the pixels the coverage test can accept: the setup record of a
+// binned face is stored and its tile span returned (0xFFFFFFFF: no pixel);
+// `giant` is set for a face over more than max_tiles tiles (not binned).
+__device__ __forceinline__ uint32_t face_setup_record(const BinJob& j, long f, const float (&v)[9], FaceSetup& s,
+                                                      bool& giant) {
+  giant = false;
+  if (!(f < j.T && setup_face(j.cam, v, j.flags, s) && s.topx >= s.botx && s.topy >= s.boty &&
+        (!face_well_conditioned(s) || tight_bbox(j.cam, s))))
+    return 0xFFFFFFFFu;
+  const int tx0 = s.botx >> kTileLog, tx1 = s.topx >> kTileLog;
+  const int ty0 = s.boty >> kTileLog, ty1 = s.topy >> kTileLog;
+  const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1;
+  if (nx * ny > j.max_tiles || nx > 16 || ny > 16) {
+    giant = true;
+    return 0xFFFFFFFFu;
+  }
+  FaceRec r;
+  r.a = make_float4(s.bx, s.by, s.cx, s.cy);
+  r.b = make_float4(s.bcnx, s.bcny, s.canx, s.cany);
+  r.c = make_float4(s.ws0, s.ws1, s.ws2, s.Az);
+  // bbox corners (16 bits each; screen < 32768 px) and bit 31: well-conditioned
+  r.d = make_float4(s.Bz, s.Cz,
+                    __int_as_float(s.botx | (s.boty << 16) | (face_well_conditioned(s) ? (int)0x80000000 : 0)),
+                    __int_as_float(s.topx | (s.topy << 16)));
+  j.recs[f] = r;
+  return pack_span(tx0, ty0, nx, ny);
+}
+
+// The tiles of span sp counted into the group's LDS histogram.
+__device__ __forceinline__ void count_span(uint32_t* cnt, int tiles_x, uint32_t sp) {
+  if (sp == 0xFFFFFFFFu) return;
+  const int tx0 = sp & 0xFFF, ty0 = (sp >> 12) & 0xFFF, nx = ((sp >> 24) & 0xF) + 1, ny = (sp >> 28) + 1;
+  for (int y = ty0; y < ty0 + ny; y++)
+    for (int x = tx0; x < tx0 + nx; x++) atomicAdd(&cnt[y * tiles_x + x], 1u);
+}
+
 __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   extern __shared__ uint32_t lds[];
   const int ntiles = j.ntiles, tiles_x = j.tiles_x, nbf = j.bin_faces;
@@ -202,61 +240,47 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   uint32_t* __restrict__ chunk_off = j.chunk_off;
   uint32_t* __restrict__ chunk_cnt = j.chunk_cnt;
   uint32_t* __restrict__ entries = j.entries;
-  FaceRec* __restrict__ recs = j.recs;
   for (int t = tid; t < ntiles; t += kBinBlock) cnt[t] = 0;
   __syncthreads();
 
-  // The positions of a lane's faces are fetched two faces ahead of their set-up,
-  // so that the loads overlap the arithmetic of the faces before them.  (A
-  // binning grid has few workgroups -- one per bin_faces faces -- so occupancy
-  // is set by the grid, not by these registers.)
-  const float* __restrict__ verts = j.verts;
-  auto fetch = [&](int i, float (&v)[9]) {
-    const long f = f0 + i;
-    if (i < nbf && f < j.T) {
-#pragma unroll
-      for (int k = 0; k < 9; k++) v[k] = verts[f * 9 + k];
+  if (j.spans) {
+    // the faces were set up by k_setup: read their spans
+    for (int i = tid; i < nbf; i += kBinBlock) {
+      const long f = f0 + i;
+      const uint32_t sp = f < j.T ? j.spans[f] : 0xFFFFFFFFu;
+      span[i] = sp;
+      count_span(cnt, tiles_x, sp);
     }
-  };
-  float p0[9], p1[9];
-  fetch(tid, p0);
-  fetch(tid + kBinBlock, p1);
-  for (int i = tid; i < nbf; i += kBinBlock) {
-    const long f = f0 + i;
-    float v[9];
+  } else {
+    // The positions of a lane's faces are fetched two faces ahead of their set-up,
+    // so that the loads overlap the arithmetic of the faces before them.
+    const float* __restrict__ verts = j.verts;
+    auto fetch = [&](int i, float (&v)[9]) {
+      const long f = f0 + i;
+      if (i < nbf && f < j.T) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      v[k] = p0[k];
-      p0[k] = p1[k];
-    }
-    fetch(i + 2 * kBinBlock, p1);
-    uint32_t sp = 0xFFFFFFFFu;
-    bool giant = false;
-    FaceSetup s;
-    if (f < j.T && setup_face(cam, v, j.flags, s) && s.topx >= s.botx && s.topy >= s.boty &&
-        (!face_well_conditioned(s) || tight_bbox(cam, s))) {
-      const int tx0 = s.botx >> kTileLog, tx1 = s.topx >> kTileLog;
-      const int ty0 = s.boty >> kTileLog, ty1 = s.topy >> kTileLog;
-      const int nx = tx1 - tx0 + 1, ny = ty1 - ty0 + 1;
-      if (nx * ny > j.max_tiles || nx > 16 || ny > 16) {
-        giant = true;
-      } else {
-        sp = pack_span(tx0, ty0, nx, ny);
-        FaceRec r;
-        r.a = make_float4(s.bx, s.by, s.cx, s.cy);
-        r.b = make_float4(s.bcnx, s.bcny, s.canx, s.cany);
-        r.c = make_float4(s.ws0, s.ws1, s.ws2, s.Az);
-        // bbox corners (16 bits each; screen < 32768 px) and bit 31: well-conditioned
-        r.d = make_float4(s.Bz, s.Cz,
-                          __int_as_float(s.botx | (s.boty << 16) | (face_well_conditioned(s) ? (int)0x80000000 : 0)),
-                          __int_as_float(s.topx | (s.topy << 16)));
-        recs[f] = r;
-        for (int y = ty0; y <= ty1; y++)
-          for (int x = tx0; x <= tx1; x++) atomicAdd(&cnt[y * tiles_x + x], 1u);
+        for (int k = 0; k < 9; k++) v[k] = verts[f * 9 + k];
       }
+    };
+    float p0[9], p1[9];
+    fetch(tid, p0);
+    fetch(tid + kBinBlock, p1);
+    for (int i = tid; i < nbf; i += kBinBlock) {
+      const long f = f0 + i;
+      float v[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        v[k] = p0[k];
+        p0[k] = p1[k];
+      }
+      fetch(i + 2 * kBinBlock, p1);
+      FaceSetup s;
+      bool giant;
+      const uint32_t sp = face_setup_record(j, f, v, s, giant);
+      count_span(cnt, tiles_x, sp);
+      span[i] = sp;
+      push_rec(giant, s, j.id_base + (uint32_t)f, j.giantq, &j.ctr->giant);
     }
-    span[i] = sp;
-    push_rec(giant, s, j.id_base + (uint32_t)f, j.giantq, &j.ctr->giant);
   }
   __syncthreads();
 
@@ -506,11 +530,29 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
 #define TINA_KERNARG_JOB(T, name, byval) const T& name = byval
 #endif
 
-__global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
+// Set-up of every face of the call, one face per lane: setup records, tile
+// spans and the giant queue for k_bin.
+__global__ void __launch_bounds__(kBlock) k_setup(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
   // the next call's counters were this call's predecessor's: every kernel of
   // that call has finished (stream order), so one thread can empty them
   if (j.ctr_next && blockIdx.x == 0 && threadIdx.x == 0) *j.ctr_next = BinCounters{0u, 0u, 0u, 0u};
+  const long f = (long)blockIdx.x * kBlock + threadIdx.x;
+  float v[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) v[k] = f < j.T ? j.verts[f * 9 + k] : 0.0f;
+  FaceSetup s;
+  bool giant;
+  const uint32_t sp = face_setup_record(j, f, v, s, giant);
+  if (f < j.T) j.spans[f] = sp;
+  push_rec(giant, s, j.id_base + (uint32_t)f, j.giantq, &j.ctr->giant);
+}
+
+__global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
+  TINA_KERNARG_JOB(BinJob, j, jv);
+  // without a set-up pass this is the call's first kernel: it empties the next
+  // call's counters (see k_setup)
+  if (j.ctr_next && !j.spans && blockIdx.x == 0 && threadIdx.x == 0) *j.ctr_next = BinCounters{0u, 0u, 0u, 0u};
   bin_group(j, blockIdx.x);
 }
 
