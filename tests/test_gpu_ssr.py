@@ -141,3 +141,70 @@ def test_mirror_floor_on_the_gpu(gpu):
     assert np.allclose(final[hit], C, rtol=1e-4, atol=0)
     assert np.all(final[miss] == 0)
     assert np.array_equal(final[wall_in], np.broadcast_to(C, (int(wall_in.sum()), 3)))
+
+
+def test_ssr_with_textured_roughness(gpu):
+    """texturing=True: the texcoord pre-shader fills the coordinate buffer and the
+    floor's roughness is a Texture sampled there (tina/postp/ssr.py:55-80 reads
+    the material at the pixel's texcoord); the reflection buffer equals the
+    oracle's on the same depth, id, normal, texcoord and image buffers."""
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import _abi
+    from taichi_three_amd.matr import lower_material
+    from test_ssr_oracle import _grid
+    print('case: ssr with a textured roughness 240x180', flush=True)
+    W, H = 240, 180
+    rng = np.random.default_rng(31)
+    scene = tina.Scene(W, H, ssr=True, tonemap=False, texturing=True, maxfaces=4000)
+    scene.engine.set_camera(tina.lookat([0, -0.2, 0], [0.2, 1.0, 3.0]), tina.perspective(50, W / H, 0.1, 30))
+    floor_v = _grid((-3, -0.7, 2), (6, 0, 0), (0, 0, -5), 16)
+    floor_n = np.broadcast_to(np.array([0, 1, 0], np.float32), floor_v.shape).copy()
+    floor_c = rng.uniform(0, 1, floor_v.shape[:2] + (2,)).astype(np.float32)
+    wall_v = _grid((-3, -0.7, -1.5), (6, 0, 0), (0, 1.5, 0), 12)
+    wall_n = np.broadcast_to(np.array([0, 0, 1], np.float32), wall_v.shape).copy()
+    wall_c = np.zeros(wall_v.shape[:2] + (2,), np.float32)
+    fl, wa = tina.SimpleMesh(), tina.SimpleMesh()
+    fl.set_face_verts(floor_v)
+    fl.set_face_norms(floor_n)
+    fl.set_face_coors(floor_c)
+    wa.set_face_verts(wall_v)
+    wa.set_face_norms(wall_n)
+    wa.set_face_coors(wall_c)
+    tex = tina.Texture(rng.uniform(0.02, 0.6, (8, 8)).astype(np.float32))
+    floor_mat = tina.CookTorrance(roughness=tex, fresnel=0.9)
+    wall_mat = tina.Lamp(color=(0.9, 0.6, 0.3))
+    scene.add_object(fl, floor_mat)
+    scene.add_object(wa, wall_mat)
+    got = _capture(scene)
+    scene.render()
+    torch.cuda.synchronize()
+    ids = scene.engine.ids.to_numpy().T.copy()
+    depth = scene.engine.depth.to_numpy().T.copy()
+    norm = scene.norm_buffer.storage.cpu().numpy()
+    coor = scene.coor_buffer.storage.cpu().numpy()
+    cam = orc.camera(scene.engine.W2V, scene.engine.V2W)
+    out = np.zeros((H, W, 4), np.float32)
+    host = []
+    base = 0
+    for hm, m in ((orc.HostMesh(floor_v, floor_n, floor_c), floor_mat),
+                  (orc.HostMesh(wall_v, wall_n, wall_c), wall_mat)):
+        hm.struct.id_base = base
+        hm.struct.nfaces = len(hm.arrays[0])
+        lo = lower_material(m, dict(attrs={}))
+        # the oracle samples the texture from a host copy of the same values
+        for i in range(lo.struct.nparams):
+            p = lo.struct.params[i]
+            if p.kind == _abi.P_TEXTURE:
+                h = np.ascontiguousarray(tex.tensor.cpu().numpy())
+                host.append(h)
+                p.ptr = h.ctypes.data
+        orc.ssr(W, H, cam, hm, lo.struct, depth, ids, norm, got['pre'], coor=coor, out=out)
+        base += hm.struct.nfaces
+    on_floor = (ids >= 0) & (ids < len(floor_v))
+    assert on_floor.sum() > 5000 and (ids >= len(floor_v)).sum() > 2000
+    assert np.abs(coor[on_floor]).max() > 0
+    assert (out[..., 3] > 0).sum() > 1000
+    d = np.abs(got['buf'] - out).max(axis=2)
+    print(f'ssr textured: max diff {d.max():.2e}, {int((d > 1e-5).sum())} pixels above 1e-5', flush=True)
+    assert (d > 1e-5).sum() <= 1e-3 * W * H
