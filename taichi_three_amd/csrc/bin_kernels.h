@@ -208,8 +208,6 @@ __device__ __forceinline__ bool tight_bbox(const Cam& cam, FaceSetup& s) {
   return true;
 }
 
-// Binning workgroup g of job j (the body of k_bin).  Needs (ntiles + bin_faces)
-// words of dynamic LDS.
 // Set-up of face f (cull, clip, bbox, edges; triangle.py:93-113) and its bbox
 // shrunk to the pixels the coverage test can accept: the setup record of a
 // binned face is stored and its tile span returned (0xFFFFFFFF: no pixel);
@@ -247,6 +245,8 @@ __device__ __forceinline__ void count_span(uint32_t* cnt, int tiles_x, uint32_t 
     for (int x = tx0; x < tx0 + nx; x++) atomicAdd(&cnt[y * tiles_x + x], 1u);
 }
 
+// Binning workgroup g of job j (the body of k_bin).  Needs (ntiles + bin_faces)
+// words of dynamic LDS.
 __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   extern __shared__ uint32_t lds[];
   const int ntiles = j.ntiles, tiles_x = j.tiles_x, nbf = j.bin_faces;
@@ -328,10 +328,6 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
       }
   }
 }
-
-// Faces too large to bin: every lane of the grid takes a slice of each one and
-// reduces into the key buffer with global atomics.  Runs after k_tile has stored
-// its tiles (a minimum does not depend on the order it is taken in).
 
 // Makes this lane's LDS writes visible to the rest of its wave before they are read.
 __device__ __forceinline__ void wave_sync_lds() {
