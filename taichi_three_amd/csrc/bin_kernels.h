@@ -614,10 +614,6 @@ k_bbin(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njob
   }
 }
 
-// The giant counts of all scenes are read at once (a batch holds at most kBlock
-// scenes) rather than one dependent load per scene.  Runs after k_btile; the
-// last workgroup to finish empties every scene's counters (scene 0's `done`
-// counts the workgroups).
 // Empties every scene's counters for the next launch of the batch (after its
 // tile pass: nothing reads them any more).
 __global__ void __launch_bounds__(kBlock) k_breset(const BinJob* __restrict__ jobs, int njobs) {
