@@ -340,10 +340,10 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
   __shared__ unsigned long long tk[kTile * kTile];
   __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
-  // per wave, per live face: its first fragment (bits 0-13) and ceil(2**16 / width) (bits 14-30)
+  // per wave, per live face: its first fragment pair (bits 0-13) and ceil(2**16 / pairs a row) (bits 14-30)
   __shared__ uint32_t Lstart[kBlock];
-  __shared__ int32_t Mark[kBlock];    // per wave: window tag at the position of a face's last fragment
-  __shared__ uint16_t Q[kBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
+  __shared__ int32_t Mark[kBlock];    // per wave: window tag at the position of a face's last fragment pair
+  __shared__ uint16_t Q[kBlock * 3];  // per wave: up to 191 pending fragments (face slot | lx | ly)
   __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
   const Cam& cam = j.cam;
@@ -400,7 +400,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       const int bx1 = min(gs.topx, ox + tw - 1), by1 = min(gs.topy, oy + th - 1);
       const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
       if (w > 0 && h > 0) {
-        area = w * h;
+        area = ((w + 1) >> 1) * h;
         ra = make_float4(gs.bx, gs.by, gs.cx, gs.cy);
         rb = make_float4(gs.bcnx, gs.bcny, gs.canx, gs.cany);
         rc = make_float4(gs.ws0, gs.ws1, gs.ws2, gs.Az);
@@ -423,7 +423,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       const int bx1 = min((int)(hi16 & 0xFFFF), ox + tw - 1), by1 = min((int)(hi16 >> 16), oy + th - 1);
       const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
       if (w > 0 && h > 0) {
-        area = w * h;
+        area = ((w + 1) >> 1) * h;
         ra = r.a;
         rb = r.b;
         rc = r.c;
@@ -433,8 +433,11 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
                          __int_as_float((int)(id_base + fid)));
       }
     }
-    // The faces with pixels in the tile take consecutive slots, so their
-    // fragment ranges [start, end) tile [0, wtotal) with no empty range.
+    // A lane takes two horizontally adjacent pixels of a face's clipped bbox (a
+    // fragment pair; the second is absent at the end of an odd row), so the
+    // face lookup below is paid once for both.  The faces with pixels in the
+    // tile take consecutive slots, so their pair ranges [start, end) tile
+    // [0, wtotal) with no empty range.
     const unsigned long long live = __ballot(area > 0);
     const int end = wave_inclusive_scan(area, lane);
     const int wtotal = __shfl(end, 63, 64);
@@ -445,19 +448,19 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       R1[slot] = rb;
       R2[slot] = rc;
       R3[slot] = rd;
-      const uint32_t wd = (uint32_t)((__float_as_int(rd.z) >> 16) & 0xFF);
-      Lstart[slot] = (uint32_t)(end - area) | (((65536u + wd - 1u) / wd) << 14);
+      const uint32_t pw = (((uint32_t)(__float_as_int(rd.z) >> 16) & 0xFF) + 1u) >> 1;  // pairs a row
+      Lstart[slot] = (uint32_t)(end - area) | (((65536u + pw - 1u) / pw) << 14);
     }
     Mark[tid] = -1;
     wave_sync_lds();
-    // Cheap pass over every fragment.  Fragment g belongs to the first face whose
-    // last fragment is >= g: per window of 64 fragments, the faces whose last
-    // fragment falls inside it tag that position, and a ballot of the tags gives
-    // each lane its face as (faces ended before the window) + (tags below it).
-    // Each fragment then evaluates the unscaled edge values and is dropped when a
-    // well-conditioned face rejects it.  Survivors (about a fifth) are compacted
-    // into the wave's LDS queue and the exact test runs on full waves of them.
-    uint16_t* const q = Q + 2 * wbase;
+    // Cheap pass over every fragment pair.  Pair g belongs to the first face whose
+    // last pair is >= g: per window of 64 pairs, the faces whose last pair falls
+    // inside it tag that position, and a ballot of the tags gives each lane its
+    // face as (faces ended before the window) + (tags below it).  Each pixel then
+    // evaluates the unscaled edge values and is dropped when a well-conditioned
+    // face rejects it.  Survivors (about two in five) are compacted into the
+    // wave's LDS queue and the exact test runs on full waves of them.
+    uint16_t* const q = Q + 3 * wbase;
     int qn = 0;     // uniform: pending fragments in the queue
     int kbase = 0;  // uniform: faces whose fragments all precede the window
     auto exact = [&](int e) {  // exact test + depth + LDS min for queue entry e
@@ -483,7 +486,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       wave_sync_lds();
       const unsigned long long ends = __ballot(Mark[tid] == g0);
       const int gI = g0 + lane;
-      bool keep = false;
+      bool keep0 = false, keep1 = false;
       int code = 0;
       if (gI < wtotal) {
         const int k = kbase + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ends >> 32),
@@ -492,33 +495,42 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
         const uint32_t ls = Lstart[j];
         const int packed = __float_as_int(R3[j].z);
         const int lx0 = packed & 0xFF, ly0 = (packed >> 8) & 0xFF, wj = (packed >> 16) & 0xFF;
+        const int pw = (wj + 1) >> 1;
         const int li = gI - (int)(ls & 0x3FFFu);
-        // li / wj as (li * ceil(2**16 / wj)) >> 16: the multiplier exceeds 2**16 / wj by
-        // less than 1, so the quotient gains less than li / 2**16 < 1/256, while li / wj
-        // sits at least 1/wj >= 1/kTile below the next integer (li < kTile^2, wj <= kTile)
+        // li / pw as (li * ceil(2**16 / pw)) >> 16: the multiplier exceeds 2**16 / pw by
+        // less than 1, so the quotient gains less than li / 2**16 < 1/512, while li / pw
+        // sits at least 1/pw >= 2/kTile below the next integer (li < kTile^2 / 2, pw <= kTile / 2)
         const int dy = (int)(__umul24((uint32_t)li, ls >> 14) >> 16);
-        const int dx = li - (int)__umul24((uint32_t)dy, (uint32_t)wj);
+        const int dx = 2 * (li - (int)__umul24((uint32_t)dy, (uint32_t)pw));
         const int lx = lx0 + dx, ly = ly0 + dy;
         const float4 r0 = R0[j], r1 = R1[j];
         FaceSetup s;
         s.bx = r0.x; s.by = r0.y; s.cx = r0.z; s.cy = r0.w;
         s.bcnx = r1.x; s.bcny = r1.y; s.canx = r1.z; s.cany = r1.w;
-        keep = !((packed >> 24) &&
-                 pixel_rejects(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1]));
+        const bool wellc = (packed >> 24) != 0;
+        const float posy = (float)(oy + ly) + cam.bias[1];
+        keep0 = !(wellc && pixel_rejects(s, (float)(ox + lx) + cam.bias[0], posy));
+        keep1 = dx + 1 < wj && !(wellc && pixel_rejects(s, (float)(ox + lx + 1) + cam.bias[0], posy));
         code = (k << (2 * kTileLog)) | (ly << kTileLog) | lx;
       }
       kbase += __popcll(ends);
-      const unsigned long long m = __ballot(keep);
-      if (keep) q[qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+      const unsigned long long m0 = __ballot(keep0), m1 = __ballot(keep1);
+      const int n0 = __popcll(m0);
+      if (keep0) q[qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u))] =
           (uint16_t)code;
-      qn += __popcll(m);
+      if (keep1)
+        q[qn + n0 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u))] =
+            (uint16_t)(code + 1);
+      qn += n0 + __popcll(m1);
       wave_sync_lds();
-      if (qn >= 64) {  // a full wave of survivors
+      while (qn >= 64) {  // a full wave of survivors (at most two per window)
         exact(lane);
-        const int rest = qn - 64;
-        const int moved = lane < rest ? q[64 + lane] : 0;
+        const int rest = qn - 64;  // <= 127
+        const int moved0 = lane < rest ? q[64 + lane] : 0;
+        const int moved1 = lane + 64 < rest ? q[128 + lane] : 0;
         wave_sync_lds();
-        if (lane < rest) q[lane] = (uint16_t)moved;
+        if (lane < rest) q[lane] = (uint16_t)moved0;
+        if (lane + 64 < rest) q[64 + lane] = (uint16_t)moved1;
         qn = rest;
         wave_sync_lds();
       }
