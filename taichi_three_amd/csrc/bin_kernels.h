@@ -147,6 +147,59 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(int n, Get v, Put out, 
   return tot;
 }
 
+// In-place exclusive scan of the LDS array a[0, n) (16-byte aligned); returns
+// the total.  As above, but each lane's run is a multiple of four values long
+// and is read and written as 16-byte vectors: single words at a stride of the
+// run length would put eight lanes on each bank.
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t* a, int n, uint32_t* wsum, uint32_t* carry) {
+  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const int per = (((n + NT - 1) / NT) + 3) & ~3;  // values per lane
+  const int i0 = tid * per;
+  uint32_t local = 0;
+  for (int k = 0; k < per; k += 4) {
+    const int i = i0 + k;
+    if (i + 3 < n) {
+      const uint4 q = *(const uint4*)(a + i);
+      local += q.x + q.y + q.z + q.w;
+    } else {
+      for (int m = 0; m < 4; m++)
+        if (i + m < n) local += a[i + m];
+    }
+  }
+  const uint32_t incl = (uint32_t)wave_inclusive_scan((int)local, lane);
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t wofs = 0, tot = 0;
+  for (int w = 0; w < NT / 64; w++) {
+    if (w < wave) wofs += wsum[w];
+    tot += wsum[w];
+  }
+  uint32_t run = wofs + incl - local;
+  for (int k = 0; k < per; k += 4) {
+    const int i = i0 + k;
+    if (i + 3 < n) {
+      const uint4 q = *(const uint4*)(a + i);
+      uint4 o;
+      o.x = run; run += q.x;
+      o.y = run; run += q.y;
+      o.z = run; run += q.z;
+      o.w = run; run += q.w;
+      *(uint4*)(a + i) = o;
+    } else {
+      for (int m = 0; m < 4; m++)
+        if (i + m < n) {
+          const uint32_t x = a[i + m];
+          a[i + m] = run;
+          run += x;
+        }
+    }
+  }
+  if (tid == 0) *carry = tot;
+  __syncthreads();
+  return tot;
+}
+
 // Shrinks the pixel bbox of a well-conditioned face to the pixels that can
 // survive the cheap rejection (and hence the coverage test); false when none can.
 //
@@ -248,7 +301,7 @@ __device__ __forceinline__ void count_span(uint32_t* cnt, int tiles_x, uint32_t 
 // Binning workgroup g of job j (the body of k_bin).  Needs (ntiles + bin_faces)
 // words of dynamic LDS.
 __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __align__(16) uint32_t lds[];
   const int ntiles = j.ntiles, tiles_x = j.tiles_x, nbf = j.bin_faces;
   const Cam& cam = j.cam;
   uint32_t* cnt = lds;            // [ntiles] counts, then running offsets
@@ -303,13 +356,8 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   }
   __syncthreads();
 
-  const uint32_t total = block_exclusive_scan<kBinBlock>(
-      ntiles, [&](int t) { return cnt[t]; },
-      [&](int t, uint32_t ofs) {
-        chunk_cnt[(size_t)g * ntiles + t] = cnt[t];
-        cnt[t] = ofs;
-      },
-      s_wsum, &s_carry);
+  for (int t = tid; t < ntiles; t += kBinBlock) chunk_cnt[(size_t)g * ntiles + t] = cnt[t];
+  const uint32_t total = block_exclusive_scan_lds<kBinBlock>(cnt, ntiles, s_wsum, &s_carry);
   if (tid == 0) s_base = atomicAdd(&j.ctr->cursor, total);
   __syncthreads();
   const uint32_t base = s_base;
