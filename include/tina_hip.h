@@ -433,7 +433,8 @@ typedef struct tina_scene_desc {
 /* The autotuned bin sizes and launch shapes. */
 typedef struct tina_batch_tuning {
     int32_t bin_faces;   /* faces per binning workgroup: 1024, 2048, 4096 or 8192      */
-    int32_t max_tiles;   /* bbox tiles (16 x 16 px) above which a face takes the giant pass */
+    int32_t max_tiles;   /* bbox tiles (16 x 16 px) above which a face is not binned: every
+                            tile of its scene takes its part from the giant queue      */
     int32_t grid_bin, grid_tile, grid_shade;  /* persistent workgroups (0: one per work item) */
 } tina_batch_tuning;
 

@@ -15,6 +15,7 @@ from .batch import BatchRenderer, BatchScenes, Tuning
 __all__ = ['candidates', 'tune', 'signature']
 
 _CACHE = {}
+LDS_BUDGET = 160 * 1024 - 1024
 
 
 def candidates(max_tiles_screen=8160, cus=256):
@@ -22,7 +23,9 @@ def candidates(max_tiles_screen=8160, cus=256):
     multiples of the CU count, 0 = one workgroup per work item)."""
     out = []
     for bf in (1024, 2048, 4096, 8192):
-        if (max_tiles_screen + bf) * 4 > 65536:
+        # a binning workgroup keeps a counter per tile of the largest screen and
+        # a span per face in LDS (the library's budget: 159 KB of the CU's 160)
+        if (max_tiles_screen + bf) * 4 > LDS_BUDGET:
             continue
         for mt in (16, 64):
             for grids in ((0, 0, 0), (2 * cus, 4 * cus, 4 * cus), (cus, 8 * cus, 8 * cus)):
