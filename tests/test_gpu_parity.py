@@ -408,3 +408,46 @@ def test_raster_random_scenes(gpu, seed):
         od, oi = orc.raster(W, H, cam, verts, flags=(1 if culling else 0) | 2)
         assert np.array_equal(ids.cpu().numpy(), oi), f'{(ids.cpu().numpy() != oi).sum()} id mismatches'
         assert np.array_equal(depth.cpu().numpy(), od)
+
+
+@pytest.mark.parametrize('seed', [0, 1, 2])
+def test_raster_random_large_calls(gpu, seed):
+    """Calls of 1.1-1.5 M faces: a binning group per CU or more, so k_bin sets
+    the faces up itself (the smaller calls above take the separate set-up pass).
+    Random screens, cameras (some inside the cloud, so the clip rule acts), face
+    sizes from sub-pixel to giant; ids and depth bit for bit against the
+    oracle's parallel raster, culling on and off."""
+    import os
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import workloads as wl
+    rng = np.random.default_rng(2000 + seed)
+    W, H = int(rng.integers(300, 1400)), int(rng.integers(200, 900))
+    n = int(rng.integers(1_100_000, 1_500_000))
+    verts = wl.triangle_soup(rng, n, np.log(rng.uniform(0.004, 0.02)), rng.uniform(0.5, 1.2))
+    big = rng.choice(n, 200, replace=False)  # a few faces over many tiles
+    verts[big] = wl.triangle_soup(rng, 200, np.log(0.8), 0.3)
+    d = rng.normal(size=3)
+    dist = float((0.4, 1.6, 3.2)[seed] * rng.uniform(0.9, 1.1))  # inside the cloud, at its edge, outside
+    view = tina.lookat(pos=tuple(rng.uniform(-0.2, 0.2, 3)), back=tuple(dist * d / np.linalg.norm(d)))
+    proj = tina.perspective(float(rng.uniform(40, 90)), W / H, 0.05, 500)
+    W2V, V2W = tina.camera_matrices(view, proj)
+    cam = orc.camera(W2V, V2W)
+    nthreads = max(1, len(os.sched_getaffinity(0)))
+    eng = tina.Engine(W, H, maxfaces=n)
+    eng.set_camera(view, proj)
+    m = tina.SimpleMesh(maxfaces=n)
+    m.set_face_verts(verts)
+    for culling in (True, False):
+        print(f'case: large call seed={seed} {W}x{H} n={n} dist={dist:.2f} culling={culling}', flush=True)
+        r = tina.TriangleRaster(eng, maxfaces=n, culling=culling)
+        eng.clear_depth()
+        r.set_object(m)
+        r.render_occup()
+        ids, depth = eng.resolve()
+        torch.cuda.synchronize()
+        keys = orc.raster_omp(W, H, cam, verts, flags=(1 if culling else 0) | 2, nthreads=nthreads)
+        od, oi = orc.unpack(keys)
+        assert (oi >= 0).sum() > 0.1 * W * H
+        assert np.array_equal(ids.cpu().numpy(), oi), f'{(ids.cpu().numpy() != oi).sum()} id mismatches'
+        assert np.array_equal(depth.cpu().numpy(), od)
