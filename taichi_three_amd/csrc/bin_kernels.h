@@ -54,6 +54,35 @@ inline int bin_faces_for(long ntiles) {
   return 0;
 }
 
+// Faces per binning group of a single raster call of nfaces faces on ncu CUs.
+// A call that fills half the CUs or more with groups of bin_faces_for(ntiles)
+// but leaves more than a tenth of them idle in its last round of one group per
+// CU is cut into equal groups, a whole number of such rounds (1.05 M faces at
+// 1080p: 256 groups of 4,096 faces instead of 128 of 8,192, half the CUs idle).
+// Every group a tile has to scan costs k_tile time, so a nearly full last
+// round is kept (2 M faces: 245 groups of 8,192; 256 of 7,813 made k_bin 4 us
+// faster and k_tile 3.5 us slower).  Smaller calls keep the largest group,
+// their faces set up in a pass of their own.
+inline int bin_faces_call(long ntiles, long nfaces, int ncu) {
+  const int bf = bin_faces_for(ntiles);
+  if (bf == 0 || ncu <= 0) return bf;
+  const long groups = (nfaces + bf - 1) / bf;
+  if (groups * 2 < ncu || groups % ncu == 0 || groups % ncu * 10 >= (long)ncu * 9) return bf;
+  const long rounds = (nfaces + (long)ncu * bf - 1) / ((long)ncu * bf);
+  const long eq = (nfaces + rounds * ncu - 1) / (rounds * ncu);
+  return (int)(eq < bf ? eq : bf);
+}
+
+// Binning groups to allocate for calls of up to cap faces: the most any call
+// of at most cap faces can use under bin_faces_call.
+inline long bin_groups_cap(long ntiles, long cap, int ncu) {
+  const int bf = bin_faces_for(ntiles);
+  if (bf == 0) return 0;
+  const long groups = (cap + bf - 1) / bf;
+  if (ncu <= 0 || groups * 2 < ncu) return groups;
+  return (cap + (long)ncu * bf - 1) / ((long)ncu * bf) * ncu;
+}
+
 // cursor: entries reserved by the binning workgroups; giant: faces in the giant
 // queue.  A single raster call alternates between two of them, k_bin emptying
 // the one the next call uses; a batch empties its scenes' after the tile pass.

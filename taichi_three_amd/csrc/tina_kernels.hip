@@ -155,7 +155,7 @@ void free_scratch(tina_ctx* c) {
 // The binned path needs its per-tile counters and the group's face spans in LDS.
 bool binned_fits(const tina_ctx* c, int nfaces) {
   const long ntiles = (long)c->tiles_x * c->tiles_y;
-  const int bf = bin_faces_for(ntiles);
+  const int bf = bin_faces_call(ntiles, nfaces, c->ncu);
   if (bf == 0) return false;
   const long groups = (nfaces + bf - 1) / bf;
   return groups <= c->groups_cap && groups <= kMaxBinGroups &&
@@ -167,7 +167,7 @@ int alloc_scratch(tina_ctx* c, int cap) {
   const size_t n = (size_t)(cap > 0 ? cap : 1);
   const size_t ntiles = (size_t)c->tiles_x * c->tiles_y;
   const int bf = bin_faces_for((long)ntiles);
-  const size_t groups = bf ? (n + bf - 1) / bf : 0;
+  const size_t groups = (size_t)bin_groups_cap((long)ntiles, (long)n, c->ncu);
   bool ok = hipMalloc(&c->bigq, sizeof(BigRec) * n) == hipSuccess &&
             hipMalloc(&c->giantq, sizeof(BigRec) * n) == hipSuccess;
   if (ok && bf && groups <= (size_t)kMaxBinGroups) {
@@ -363,6 +363,12 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
     release();
     return fail(nullptr, TINA_ERR_HIP, "counter initialisation failed");
   }
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      c->ncu = 256;
+    (void)hipGetLastError();
+  }
   int rc = alloc_scratch(c, max_faces);
   if (rc != TINA_OK) {
     g_create_err = c->err;
@@ -370,12 +376,6 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
     return rc;
   }
   c->nn_grid = resident_grid((const void*)k_shade_nn, kBlock, 768);
-  {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      c->ncu = 256;
-    (void)hipGetLastError();
-  }
   *out = c;
   return TINA_OK;
 }
@@ -448,7 +448,7 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.flags = flags;
     j.tiles_x = c->tiles_x;
     j.ntiles = c->tiles_x * c->tiles_y;
-    j.bin_faces = bin_faces_for(j.ntiles);
+    j.bin_faces = bin_faces_call(j.ntiles, nfaces, c->ncu);
     j.ngroups = (nfaces + j.bin_faces - 1) / j.bin_faces;
     j.max_tiles = kBinMaxTiles;
     // Few binning groups leave most CUs idle while each group sets its faces up
