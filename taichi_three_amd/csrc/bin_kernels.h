@@ -83,12 +83,8 @@ inline long bin_groups_cap(long ntiles, long cap, int ncu) {
   return (cap + (long)ncu * bf - 1) / ((long)ncu * bf) * ncu;
 }
 
-// cursor: entries reserved by the binning workgroups; giant: faces in the giant
-// queue.  A single raster call alternates between two of them, k_bin emptying
-// the one the next call uses; a batch empties its scenes' after the tile pass.
-struct BinCounters {
-  uint32_t cursor, giant, done, pad;
-};
+// A face span value k_setup writes for a giant face (its set-up is in giantq).
+constexpr uint32_t kSpanGiant = 0xFFFFFFFEu;
 
 // Setup record of a binned face (64 B, one cache line): the state the coverage
 // loop needs (triangle.py:127-131 plus the view-space z of A, B, C) and its bbox.
@@ -111,9 +107,15 @@ struct BinJob {
   uint32_t* chunk_cnt;
   uint32_t* entries;
   FaceRec* recs;
-  BigRec* giantq;
-  BinCounters* ctr;
-  BinCounters* ctr_next;  // zeroed for the next call (counters alternate), or null
+  // Every call leaves the state the next one starts from, so a call captured in
+  // a graph replays as it ran: the entry cursor (binning groups reserve their
+  // entries with one atomic each) is emptied by the tile pass, which does not
+  // read it, and each binning group lists its giant faces at
+  // gids[g * bin_faces, + gcnt[g]) instead of appending them to a shared queue.
+  uint32_t* cursor; // entries reserved so far by the call's binning groups
+  BigRec* giantq;   // [T] the set-up of each giant face, at its face index
+  uint32_t* gids;   // [T] giant face ids by group (see above)
+  uint32_t* gcnt;   // [ngroups] giant faces of each binning group
   uint32_t* spans;        // [T] per-face tile spans written by k_setup, or null: k_bin sets the faces up
   Cam cam;
   int T;
@@ -319,6 +321,15 @@ __device__ __forceinline__ uint32_t face_setup_record(const BinJob& j, long f, c
   return pack_span(tx0, ty0, nx, ny);
 }
 
+// The set-up of giant face f, at its face index (k_tile reads it through gids).
+__device__ __forceinline__ void store_giant(const BinJob& j, long f, const FaceSetup& s) {
+  BigRec r;
+  r.s = s;
+  r.id = j.id_base + (uint32_t)f;
+  r.pad = 0;
+  j.giantq[f] = r;
+}
+
 // The tiles of span sp counted into the group's LDS histogram.
 __device__ __forceinline__ void count_span(uint32_t* cnt, int tiles_x, uint32_t sp) {
   if (sp == 0xFFFFFFFFu) return;
@@ -335,20 +346,26 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   const Cam& cam = j.cam;
   uint32_t* cnt = lds;            // [ntiles] counts, then running offsets
   uint32_t* span = lds + ntiles;  // [bin_faces] per-face tile span
-  __shared__ uint32_t s_base, s_carry, s_wsum[kBinBlock / 64];
+  __shared__ uint32_t s_base, s_carry, s_gcnt, s_wsum[kBinBlock / 64];
   const int tid = threadIdx.x;
   const long f0 = (long)g * nbf;
   uint32_t* __restrict__ chunk_off = j.chunk_off;
   uint32_t* __restrict__ chunk_cnt = j.chunk_cnt;
   uint32_t* __restrict__ entries = j.entries;
+  uint32_t* __restrict__ gids = j.gids + f0;
   for (int t = tid; t < ntiles; t += kBinBlock) cnt[t] = 0;
+  if (tid == 0) s_gcnt = 0;
   __syncthreads();
 
   if (j.spans) {
-    // the faces were set up by k_setup: read their spans
+    // the faces were set up by k_setup: read their spans, list the giants
     for (int i = tid; i < nbf; i += kBinBlock) {
       const long f = f0 + i;
-      const uint32_t sp = f < j.T ? j.spans[f] : 0xFFFFFFFFu;
+      uint32_t sp = f < j.T ? j.spans[f] : 0xFFFFFFFFu;
+      if (sp == kSpanGiant) {
+        gids[atomicAdd(&s_gcnt, 1u)] = (uint32_t)f;
+        sp = 0xFFFFFFFFu;
+      }
       span[i] = sp;
       count_span(cnt, tiles_x, sp);
     }
@@ -380,14 +397,18 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
       const uint32_t sp = face_setup_record(j, f, v, s, giant);
       count_span(cnt, tiles_x, sp);
       span[i] = sp;
-      push_rec(giant, s, j.id_base + (uint32_t)f, j.giantq, &j.ctr->giant);
+      if (giant) {
+        store_giant(j, f, s);
+        gids[atomicAdd(&s_gcnt, 1u)] = (uint32_t)f;
+      }
     }
   }
   __syncthreads();
+  if (tid == 0) j.gcnt[g] = s_gcnt;
 
   for (int t = tid; t < ntiles; t += kBinBlock) chunk_cnt[(size_t)g * ntiles + t] = cnt[t];
   const uint32_t total = block_exclusive_scan_lds<kBinBlock>(cnt, ntiles, s_wsum, &s_carry);
-  if (tid == 0) s_base = atomicAdd(&j.ctr->cursor, total);
+  if (tid == 0) s_base = atomicAdd(j.cursor, total);
   __syncthreads();
   const uint32_t base = s_base;
   for (int t = tid; t < ntiles; t += kBinBlock) chunk_off[(size_t)g * ntiles + t] = base + cnt[t];
@@ -406,21 +427,30 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   }
 }
 
+// Dynamic LDS of tile_body for a job of ngroups binning groups.
+inline size_t tile_lds_bytes(int ngroups) { return (size_t)(ngroups + 1) * 4 + ((size_t)ngroups * 2 + 3) / 4 * 4; }
+
 // Makes this lane's LDS writes visible to the rest of its wave before they are read.
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
-// Tile t of job j (the body of k_tile).  Needs ngroups + 1 words of dynamic LDS.
+// Tile t of job j (the body of k_tile).  Needs tile_lds_bytes(ngroups) of dynamic LDS.
 __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
-  extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of the tile's chunk counts
+  // per binning group: its faces for this tile, the chunk's binned faces then the
+  // group's giant faces (each tile takes the part of every giant inside it)
+  extern __shared__ uint32_t pre[];  // [ngroups + 1] exclusive prefix of those counts
+  // [ngroups] the groups' giant counts (at most bin_faces each); 16 bits, as the
+  // tags above, keep seven workgroups per CU up to 336 groups
+  uint16_t* const gc = (uint16_t*)(pre + j.ngroups + 1);
   __shared__ unsigned long long tk[kTile * kTile];
   __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
   // per wave, per live face: its first fragment pair (bits 0-13) and ceil(2**16 / pairs a row) (bits 14-30)
   __shared__ uint32_t Lstart[kBlock];
-  __shared__ int32_t Mark[kBlock];    // per wave: window tag at the position of a face's last fragment pair
-  __shared__ uint16_t Q[kBlock * 3];  // per wave: up to 191 pending fragments (face slot | lx | ly)
+  // per wave: window tag (a window start below 2**13) at the position of a face's last fragment pair
+  __shared__ int16_t Mark[kBlock];
+  __shared__ uint16_t Q[kBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
   __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
   const Cam& cam = j.cam;
@@ -436,13 +466,14 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   const int ntiles = j.ntiles;  // chunk tables are group-major: [group][tile]
   // (Staging the chunks' entry offsets in LDS with the counts, to take one
   // dependent read out of the face loop, measured no faster.)
-  const uint32_t nbin = block_exclusive_scan<kBlock>(
-      ngroups, [&](int c) { return chunk_cnt[(size_t)c * ntiles + t]; },
+  const uint32_t total = block_exclusive_scan<kBlock>(
+      ngroups,
+      [&](int c) {
+        const uint32_t ng = j.gcnt[c];
+        gc[c] = (uint16_t)ng;
+        return chunk_cnt[(size_t)c * ntiles + t] + ng;
+      },
       [&](int c, uint32_t ofs) { pre[c] = ofs; }, s_wsum, &s_carry);
-  // the giant faces (not binned: more than max_tiles tiles) follow the binned
-  // ones in the tile's list; each tile takes the part of every giant inside it
-  const uint32_t ngiant = __atomic_load_n(&j.ctr->giant, __ATOMIC_RELAXED);
-  const uint32_t total = nbin + ngiant;
   const int tw = min(kTile, cam.W - ox), th = min(kTile, cam.H - oy);
   if (total == 0) {
     if (!j.load_keys)  // the deferred clear of a tile no face touches
@@ -453,7 +484,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     return;
   }
   if (tid == 0) {
-    pre[ngroups] = nbin;
+    pre[ngroups] = total;
     s_next = 0;
   }
   for (int i = tid; i < kTile * kTile; i += kBlock) {
@@ -470,44 +501,47 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     const uint32_t v = b0 + lane;
     int area = 0;
     float4 ra, rb, rc, rd;
-    if (v >= nbin && v < total) {
-      const BigRec& g = j.giantq[v - nbin];
-      const FaceSetup& gs = g.s;
-      const int bx0 = max(gs.botx, ox), by0 = max(gs.boty, oy);
-      const int bx1 = min(gs.topx, ox + tw - 1), by1 = min(gs.topy, oy + th - 1);
-      const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
-      if (w > 0 && h > 0) {
-        area = ((w + 1) >> 1) * h;
-        ra = make_float4(gs.bx, gs.by, gs.cx, gs.cy);
-        rb = make_float4(gs.bcnx, gs.bcny, gs.canx, gs.cany);
-        rc = make_float4(gs.ws0, gs.ws1, gs.ws2, gs.Az);
-        rd = make_float4(gs.Bz, gs.Cz,
-                         __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) |
-                                        ((face_well_conditioned(gs) ? 1 : 0) << 24)),
-                         __int_as_float((int)g.id));
-      }
-    } else if (v < total) {
+    if (v < total) {
       int lo = 0, hi = ngroups;  // pre[lo] <= v < pre[hi]
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (pre[mid] <= v) lo = mid; else hi = mid;
       }
-      const uint32_t fid = entries[chunk_off[(size_t)lo * ntiles + t] + (v - pre[lo])];
-      const FaceRec r = recs[fid];
-      const uint32_t lo16 = (uint32_t)__float_as_int(r.d.z), hi16 = (uint32_t)__float_as_int(r.d.w);
-      const int wellc = (int)(lo16 >> 31);
-      const int bx0 = max((int)(lo16 & 0xFFFF), ox), by0 = max((int)((lo16 >> 16) & 0x7FFF), oy);
-      const int bx1 = min((int)(hi16 & 0xFFFF), ox + tw - 1), by1 = min((int)(hi16 >> 16), oy + th - 1);
-      const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
-      if (w > 0 && h > 0) {
-        area = ((w + 1) >> 1) * h;
-        ra = r.a;
-        rb = r.b;
-        rc = r.c;
-        // Bz, Cz, clipped origin (x | y << 8), width, well-conditioned bit; face id
-        rd = make_float4(r.d.x, r.d.y,
-                         __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) | (wellc << 24)),
-                         __int_as_float((int)(id_base + fid)));
+      const uint32_t i = v - pre[lo], nb = pre[lo + 1] - pre[lo] - gc[lo];
+      if (i < nb) {
+        const uint32_t fid = entries[chunk_off[(size_t)lo * ntiles + t] + i];
+        const FaceRec r = recs[fid];
+        const uint32_t lo16 = (uint32_t)__float_as_int(r.d.z), hi16 = (uint32_t)__float_as_int(r.d.w);
+        const int wellc = (int)(lo16 >> 31);
+        const int bx0 = max((int)(lo16 & 0xFFFF), ox), by0 = max((int)((lo16 >> 16) & 0x7FFF), oy);
+        const int bx1 = min((int)(hi16 & 0xFFFF), ox + tw - 1), by1 = min((int)(hi16 >> 16), oy + th - 1);
+        const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
+        if (w > 0 && h > 0) {
+          area = ((w + 1) >> 1) * h;
+          ra = r.a;
+          rb = r.b;
+          rc = r.c;
+          // Bz, Cz, clipped origin (x | y << 8), width, well-conditioned bit; face id
+          rd = make_float4(r.d.x, r.d.y,
+                           __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) | (wellc << 24)),
+                           __int_as_float((int)(id_base + fid)));
+        }
+      } else {
+        const BigRec& g = j.giantq[j.gids[(size_t)lo * j.bin_faces + (i - nb)]];
+        const FaceSetup& gs = g.s;
+        const int bx0 = max(gs.botx, ox), by0 = max(gs.boty, oy);
+        const int bx1 = min(gs.topx, ox + tw - 1), by1 = min(gs.topy, oy + th - 1);
+        const int w = bx1 - bx0 + 1, h = by1 - by0 + 1;
+        if (w > 0 && h > 0) {
+          area = ((w + 1) >> 1) * h;
+          ra = make_float4(gs.bx, gs.by, gs.cx, gs.cy);
+          rb = make_float4(gs.bcnx, gs.bcny, gs.canx, gs.cany);
+          rc = make_float4(gs.ws0, gs.ws1, gs.ws2, gs.Az);
+          rd = make_float4(gs.Bz, gs.Cz,
+                           __int_as_float((bx0 - ox) | ((by0 - oy) << 8) | (w << 16) |
+                                          ((face_well_conditioned(gs) ? 1 : 0) << 24)),
+                           __int_as_float((int)g.id));
+        }
       }
     }
     // A lane takes two horizontally adjacent pixels of a face's clipped bbox (a
@@ -537,7 +571,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     // evaluates the unscaled edge values and is dropped when a well-conditioned
     // face rejects it.  Survivors (about two in five) are compacted into the
     // wave's LDS queue and the exact test runs on full waves of them.
-    uint16_t* const q = Q + 3 * wbase;
+    uint16_t* const q = Q + 2 * wbase;
     int qn = 0;     // uniform: pending fragments in the queue
     int kbase = 0;  // uniform: faces whose fragments all precede the window
     auto exact = [&](int e) {  // exact test + depth + LDS min for queue entry e
@@ -559,7 +593,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       }
     };
     for (int g0 = 0; g0 < wtotal; g0 += 64) {
-      if (area > 0 && end - 1 >= g0 && end - 1 < g0 + 64) Mark[wbase + end - 1 - g0] = g0;
+      if (area > 0 && end - 1 >= g0 && end - 1 < g0 + 64) Mark[wbase + end - 1 - g0] = (int16_t)g0;
       wave_sync_lds();
       const unsigned long long ends = __ballot(Mark[tid] == g0);
       const int gI = g0 + lane;
@@ -591,25 +625,25 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
         code = (k << (2 * kTileLog)) | (ly << kTileLog) | lx;
       }
       kbase += __popcll(ends);
-      const unsigned long long m0 = __ballot(keep0), m1 = __ballot(keep1);
-      const int n0 = __popcll(m0);
-      if (keep0) q[qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u))] =
-          (uint16_t)code;
-      if (keep1)
-        q[qn + n0 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u))] =
-            (uint16_t)(code + 1);
-      qn += n0 + __popcll(m1);
-      wave_sync_lds();
-      while (qn >= 64) {  // a full wave of survivors (at most two per window)
-        exact(lane);
-        const int rest = qn - 64;  // <= 127
-        const int moved0 = lane < rest ? q[64 + lane] : 0;
-        const int moved1 = lane + 64 < rest ? q[128 + lane] : 0;
+      // the first pixels of the pairs, then the second ones: each push adds at
+      // most 64 to at most 63 pending, and a full wave is drained after each
+      for (int half = 0; half < 2; half++) {
+        const bool keep = half ? keep1 : keep0;
+        const unsigned long long m = __ballot(keep);
+        if (keep)
+          q[qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+              (uint16_t)(code + half);
+        qn += __popcll(m);
         wave_sync_lds();
-        if (lane < rest) q[lane] = (uint16_t)moved0;
-        if (lane + 64 < rest) q[64 + lane] = (uint16_t)moved1;
-        qn = rest;
-        wave_sync_lds();
+        if (qn >= 64) {  // a full wave of survivors
+          exact(lane);
+          const int rest = qn - 64;  // <= 63
+          const int moved = lane < rest ? q[64 + lane] : 0;
+          wave_sync_lds();
+          if (lane < rest) q[lane] = (uint16_t)moved;
+          qn = rest;
+          wave_sync_lds();
+        }
       }
     }
     if (lane < qn) exact(lane);
@@ -635,12 +669,9 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
 #endif
 
 // Set-up of every face of the call, one face per lane: setup records, tile
-// spans and the giant queue for k_bin.
+// spans (kSpanGiant for a giant face, whose set-up goes to giantq) for k_bin.
 __global__ void __launch_bounds__(kBlock) k_setup(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
-  // the next call's counters were this call's predecessor's: every kernel of
-  // that call has finished (stream order), so one thread can empty them
-  if (j.ctr_next && blockIdx.x == 0 && threadIdx.x == 0) *j.ctr_next = BinCounters{0u, 0u, 0u, 0u};
   const long f = (long)blockIdx.x * kBlock + threadIdx.x;
   float v[9];
 #pragma unroll
@@ -648,15 +679,14 @@ __global__ void __launch_bounds__(kBlock) k_setup(const BinJob jv) {
   FaceSetup s;
   bool giant;
   const uint32_t sp = face_setup_record(j, f, v, s, giant);
-  if (f < j.T) j.spans[f] = sp;
-  push_rec(giant, s, j.id_base + (uint32_t)f, j.giantq, &j.ctr->giant);
+  if (giant) store_giant(j, f, s);
+  if (f < j.T) j.spans[f] = giant ? kSpanGiant : sp;
 }
 
 __global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
   // without a set-up pass this is the call's first kernel: it empties the next
   // call's counters (see k_setup)
-  if (j.ctr_next && !j.spans && blockIdx.x == 0 && threadIdx.x == 0) *j.ctr_next = BinCounters{0u, 0u, 0u, 0u};
   bin_group(j, blockIdx.x);
 }
 
@@ -675,6 +705,9 @@ __device__ __forceinline__ long strip_item(long b, long n) {
 
 __global__ void __launch_bounds__(kBlock) k_tile(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
+  // the binning groups have all reserved their entries (stream order) and no tile
+  // reads the cursor: empty it for the next call
+  if (blockIdx.x == 0 && threadIdx.x == 0) *j.cursor = 0u;
   tile_body(j, (int)strip_item(blockIdx.x, gridDim.x));
 }
 
@@ -703,14 +736,11 @@ k_bbin(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njob
   }
 }
 
-// Empties every scene's counters for the next launch of the batch (after its
-// tile pass: nothing reads them any more).
-__global__ void __launch_bounds__(kBlock) k_breset(const BinJob* __restrict__ jobs, int njobs) {
-  for (int s = threadIdx.x; s < njobs; s += kBlock) *jobs[s].ctr = BinCounters{0u, 0u, 0u, 0u};
-}
-
 __global__ void __launch_bounds__(kBlock)
 k_btile(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
+  // every scene's binning is done and no tile reads the cursors: empty them
+  if (blockIdx.x == 0)
+    for (int s = threadIdx.x; s < njobs; s += kBlock) *jobs[s].cursor = 0u;
   const long total = prefix[njobs];
   for (long b = blockIdx.x; b < total; b += gridDim.x) {
     const long sweep = b - blockIdx.x, width = min((long)gridDim.x, total - sweep);

@@ -35,15 +35,17 @@ struct tina_ctx {
   BigRec* bigq = nullptr;
   BigRec* giantq = nullptr;
   Counters* ctr = nullptr;
-  // binned path: per (tile, binning group) chunk records, the bin entries, counters
+  // binned path: per (tile, binning group) chunk records, the bin entries, the
+  // giant faces by group and each group's giant count
   uint32_t* chunk_off = nullptr;
   uint32_t* chunk_cnt = nullptr;
   uint32_t* entries = nullptr;
+  uint32_t* gids = nullptr;
+  uint32_t* gcnt = nullptr;
+  uint32_t* cursor = nullptr;     // the binning groups' entry cursor (zero between calls)
   FaceRec* recs = nullptr;
   uint32_t* spans = nullptr;      // per-face tile spans (k_setup -> k_bin)
   int ncu = 0;                    // compute units of the device
-  BinCounters* binctr = nullptr;  // two: a raster call uses one and empties the other
-  int bin_parity = 0;
   // learned-shading fallback queue (kFbSeg segments of fb_segcap pixel indices),
   // two buffers of segment counters, and k_shade_nn's resident grid
   uint32_t* fbq = nullptr;
@@ -143,10 +145,10 @@ int flush_clear(tina_ctx* c, hipStream_t stream) {
 
 void free_scratch(tina_ctx* c) {
   for (void* p : {(void*)c->bigq, (void*)c->giantq, (void*)c->chunk_off, (void*)c->chunk_cnt, (void*)c->entries,
-                  (void*)c->recs, (void*)c->spans})
+                  (void*)c->gids, (void*)c->gcnt, (void*)c->recs, (void*)c->spans})
     if (p) (void)hipFree(p);
   c->bigq = c->giantq = nullptr;
-  c->chunk_off = c->chunk_cnt = c->entries = nullptr;
+  c->chunk_off = c->chunk_cnt = c->entries = c->gids = c->gcnt = nullptr;
   c->recs = nullptr;
   c->spans = nullptr;
   c->groups_cap = 0;
@@ -159,7 +161,8 @@ bool binned_fits(const tina_ctx* c, int nfaces) {
   if (bf == 0) return false;
   const long groups = (nfaces + bf - 1) / bf;
   return groups <= c->groups_cap && groups <= kMaxBinGroups &&
-         c->W < 32768 && c->H < 32768;  // FaceRec packs bbox corners in 16 bits
+         (long)nfaces * kBinMaxTiles <= 0xFFFFFFFFL &&  // 32-bit entry indices
+         c->W < 32768 && c->H < 32768;                  // FaceRec packs bbox corners in 16 bits
 }
 
 int alloc_scratch(tina_ctx* c, int cap) {
@@ -174,6 +177,8 @@ int alloc_scratch(tina_ctx* c, int cap) {
     ok = hipMalloc(&c->chunk_off, 4 * ntiles * groups) == hipSuccess &&
          hipMalloc(&c->chunk_cnt, 4 * ntiles * groups) == hipSuccess &&
          hipMalloc(&c->entries, 4 * n * kBinMaxTiles) == hipSuccess &&
+         hipMalloc(&c->gids, sizeof(uint32_t) * n) == hipSuccess &&
+         hipMalloc(&c->gcnt, sizeof(uint32_t) * groups) == hipSuccess &&
          hipMalloc(&c->recs, sizeof(FaceRec) * n) == hipSuccess &&
          hipMalloc(&c->spans, sizeof(uint32_t) * n) == hipSuccess;
     if (ok) c->groups_cap = (int)groups;
@@ -341,7 +346,7 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
   c->cam.W2V[10] = c->cam.V2W[10] = -1.0f;
   c->cam.bias[0] = c->cam.bias[1] = 0.5f;
   auto release = [c]() {
-    for (void* p : {(void*)c->keys, (void*)c->ctr, (void*)c->binctr, (void*)c->fbq, (void*)c->fbn})
+    for (void* p : {(void*)c->keys, (void*)c->ctr, (void*)c->cursor, (void*)c->fbq, (void*)c->fbn})
       if (p) (void)hipFree(p);
     delete c;
   };
@@ -349,15 +354,14 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
   const long nchunks = ((long)width * height + kNnShadePix - 1) / kNnShadePix;
   c->fb_segcap = (nchunks + kFbSeg - 1) / kFbSeg * kNnShadePix;
   if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
-      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->binctr, 2 * sizeof(BinCounters)) != hipSuccess ||
+      hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->cursor, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->fbq, sizeof(uint32_t) * (size_t)c->fb_segcap * kFbSeg) != hipSuccess ||
       hipMalloc(&c->fbn, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess) {
     (void)hipGetLastError();
     release();
     return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the depth|id buffer failed");
   }
-  if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess ||
-      hipMemset(c->binctr, 0, 2 * sizeof(BinCounters)) != hipSuccess ||
+  if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess || hipMemset(c->cursor, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->fbn, 0, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     release();
@@ -383,7 +387,7 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
 int tina_ctx_destroy(tina_ctx* c) {
   if (!c) return TINA_OK;
   free_scratch(c);
-  for (void* p : {(void*)c->keys, (void*)c->ctr, (void*)c->binctr, (void*)c->fbq, (void*)c->fbn})
+  for (void* p : {(void*)c->keys, (void*)c->ctr, (void*)c->cursor, (void*)c->fbq, (void*)c->fbn})
     if (p) (void)hipFree(p);
   delete c;
   return TINA_OK;
@@ -438,10 +442,10 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.chunk_cnt = c->chunk_cnt;
     j.entries = c->entries;
     j.recs = c->recs;
+    j.cursor = c->cursor;
     j.giantq = c->giantq;
-    j.ctr = c->binctr + c->bin_parity;
-    j.ctr_next = c->binctr + (c->bin_parity ^ 1);
-    c->bin_parity ^= 1;
+    j.gids = c->gids;
+    j.gcnt = c->gcnt;
     j.cam = cam;
     j.T = nfaces;
     j.id_base = id_base;
@@ -468,7 +472,7 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     }
     hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + j.bin_faces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
-    hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), (size_t)(j.ngroups + 1) * 4, s, j);
+    hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), tile_lds_bytes(j.ngroups), s, j);
     TINA_LAUNCH(c, "k_tile");
     return TINA_OK;
   }
@@ -1091,24 +1095,28 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     const int tx = (d.width + kTile - 1) / kTile, ty = (d.height + kTile - 1) / kTile;
     const int ntiles = tx * ty;
     const int ngroups = (d.nfaces + bf - 1) / bf;
-    if ((ntiles + bf) * 4 > kLdsBudget || ngroups > kMaxBinGroups) {
+    if ((ntiles + bf) * 4 > kLdsBudget || ngroups > kMaxBinGroups ||
+        (long)d.nfaces * tune->max_tiles > 0xFFFFFFFFL) {
       batch_free(b);
-      return fail(nullptr, TINA_ERR_CAPACITY, "scene " + std::to_string(i) + ": screen too large for bin_faces");
+      return fail(nullptr, TINA_ERR_CAPACITY, "scene " + std::to_string(i) + ": screen or face count too large for the binned path");
     }
     b->max_ntiles = std::max(b->max_ntiles, ntiles);
     b->max_groups = std::max(b->max_groups, ngroups);
     b->key_off[i] = b->total_pix;
     b->total_pix += (long)d.width * d.height;
     const size_t T = (size_t)std::max(d.nfaces, 1);
-    void *co = nullptr, *cc = nullptr, *en = nullptr, *rc_ = nullptr, *gq = nullptr, *ct = nullptr;
+    void *co = nullptr, *cc = nullptr, *en = nullptr, *rc_ = nullptr, *gq = nullptr, *gi = nullptr, *gn = nullptr,
+         *cu = nullptr;
     bool ok = hipMalloc(&co, 4 * (size_t)ntiles * std::max(ngroups, 1)) == hipSuccess &&
               hipMalloc(&cc, 4 * (size_t)ntiles * std::max(ngroups, 1)) == hipSuccess &&
               hipMalloc(&en, 4 * T * (size_t)tune->max_tiles) == hipSuccess &&
               hipMalloc(&rc_, sizeof(FaceRec) * T) == hipSuccess && hipMalloc(&gq, sizeof(BigRec) * T) == hipSuccess &&
-              hipMalloc(&ct, sizeof(BinCounters)) == hipSuccess;
-    for (void* p : {co, cc, en, rc_, gq, ct})
+              hipMalloc(&gi, sizeof(uint32_t) * T) == hipSuccess &&
+              hipMalloc(&gn, sizeof(uint32_t) * (size_t)std::max(ngroups, 1)) == hipSuccess &&
+              hipMalloc(&cu, sizeof(uint32_t)) == hipSuccess;
+    for (void* p : {co, cc, en, rc_, gq, gi, gn, cu})
       if (p) b->allocs.push_back(p);
-    if (!ok || hipMemset(ct, 0, sizeof(BinCounters)) != hipSuccess) {
+    if (!ok || hipMemset(cu, 0, sizeof(uint32_t)) != hipSuccess) {
       (void)hipGetLastError();
       batch_free(b);
       return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the batch scratch failed");
@@ -1120,9 +1128,10 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     j.chunk_cnt = (uint32_t*)cc;
     j.entries = (uint32_t*)en;
     j.recs = (FaceRec*)rc_;
+    j.cursor = (uint32_t*)cu;
     j.giantq = (BigRec*)gq;
-    j.ctr = (BinCounters*)ct;
-    j.ctr_next = nullptr;
+    j.gids = (uint32_t*)gi;
+    j.gcnt = (uint32_t*)gn;
     j.spans = nullptr;  // k_bbin sets its faces up itself
     memcpy(j.cam.W2V, d.cam.W2V, sizeof(j.cam.W2V));
     memcpy(j.cam.V2W, d.cam.V2W, sizeof(j.cam.V2W));
@@ -1201,10 +1210,8 @@ int tina_batch_render(tina_batch* b, void* stream) {
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_bbin"))) return rc;
   // every tile of every scene starts from empty keys and is written whole: no clear pass
   hipLaunchKernelGGL(k_btile, dim3(batch_grid(b->tune.grid_tile, b->totals[1])), dim3(kBlock),
-                     (size_t)(b->max_groups + 1) * 4, s, b->d_jobs, b->d_prefix + (n + 1), n);
+                     tile_lds_bytes(b->max_groups), s, b->d_jobs, b->d_prefix + (n + 1), n);
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_btile"))) return rc;
-  hipLaunchKernelGGL(k_breset, dim3(1), dim3(kBlock), 0, s, b->d_jobs, n);
-  if ((rc = hip_check(nullptr, hipGetLastError(), "k_breset"))) return rc;
   const unsigned gs = batch_grid(b->tune.grid_shade, b->totals[2]);
   const int* pre = b->d_prefix + 2 * (n + 1);
   switch (b->prog) {
