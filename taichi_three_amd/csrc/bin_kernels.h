@@ -56,18 +56,16 @@ inline int bin_faces_for(long ntiles) {
 
 // Faces per binning group of a single raster call of nfaces faces on ncu CUs.
 // A call that fills half the CUs or more with groups of bin_faces_for(ntiles)
-// but leaves more than a tenth of them idle in its last round of one group per
-// CU is cut into equal groups, a whole number of such rounds (1.05 M faces at
-// 1080p: 256 groups of 4,096 faces instead of 128 of 8,192, half the CUs idle).
-// Every group a tile has to scan costs k_tile time, so a nearly full last
-// round is kept (2 M faces: 245 groups of 8,192; 256 of 7,813 made k_bin 4 us
-// faster and k_tile 3.5 us slower).  Smaller calls keep the largest group,
-// their faces set up in a pass of their own.
+// is cut into equal groups, a whole number of rounds of one group per CU, so
+// that no CU waits while others finish a partial last round (1.1 M faces at
+// 1080p: 256 groups of 4,297 faces instead of 135 of 8,192; 2 M faces: 256 of
+// 7,813 instead of 245 of 8,192).  Smaller calls keep the largest group, their
+// faces set up in a pass of their own.
 inline int bin_faces_call(long ntiles, long nfaces, int ncu) {
   const int bf = bin_faces_for(ntiles);
   if (bf == 0 || ncu <= 0) return bf;
   const long groups = (nfaces + bf - 1) / bf;
-  if (groups * 2 < ncu || groups % ncu == 0 || groups % ncu * 10 >= (long)ncu * 9) return bf;
+  if (groups * 2 < ncu || groups % ncu == 0) return bf;
   const long rounds = (nfaces + (long)ncu * bf - 1) / ((long)ncu * bf);
   const long eq = (nfaces + rounds * ncu - 1) / (rounds * ncu);
   return (int)(eq < bf ? eq : bf);

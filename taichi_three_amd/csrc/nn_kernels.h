@@ -38,9 +38,12 @@ constexpr int kNnPackBias = kNnFrags * 64 * 8;  // offset of the biases, in bf16
 constexpr int kNnPackBytes = kNnFrags * 64 * 16 + (3 * 64 + 4) * 4;
 // Fallback queue of k_shade_nn: pixel chunk c appends to segment c % kFbSeg,
 // whose counter sits alone on a 128-byte line (one global atomic per chunk).
-// Per buffer: kFbSeg counters kFbLine words apart, then the last pass's total.
+// kFbSeg counters kFbLine words apart, then the last pass's total, then the
+// ticket k_shade_list's workgroups take when they finish: the last one empties
+// the counters and the ticket, so every call starts from zeros (a captured call
+// replays as it ran).
 constexpr int kFbSeg = 32, kFbLine = 32;
-constexpr int kFbWords = (kFbSeg + 1) * kFbLine;
+constexpr int kFbWords = (kFbSeg + 2) * kFbLine;
 
 // ---- random inputs -----------------------------------------------------------
 
@@ -562,13 +565,14 @@ __device__ __forceinline__ bf16x8 nn_next_b(const f32x4& lo, const f32x4& hi) {
 }
 
 // The analytic shader over the pixels k_shade_nn queued: segment k holds
-// fbn[k * kFbLine] pixels at fbq + k * segcap.  Also clears fbn_next, the
-// counters the next k_shade_nn launch appends to, and stores the total after
-// the counters (fbn[kFbSeg * kFbLine]).
+// fbn[k * kFbLine] pixels at fbq + k * segcap.  Stores the total after the
+// counters (fbn[kFbSeg * kFbLine]); the last workgroup to finish empties the
+// counters for the next k_shade_nn launch (every workgroup has read them
+// before it takes its ticket).
 template <int PROG, bool IBL>
 __global__ void __launch_bounds__(kBlock)
 k_shade_list(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ fbq,
-             long segcap, uint32_t* __restrict__ fbn, uint32_t* __restrict__ fbn_next, float* __restrict__ img) {
+             long segcap, uint32_t* __restrict__ fbn, float* __restrict__ img) {
   (void)a_by_value;
 #if defined(__HIP_DEVICE_COMPILE__)
   const ShadeArgs& a = *(const ShadeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -585,10 +589,7 @@ k_shade_list(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, cons
   }
   __syncthreads();
   const uint32_t total = s_pre[kFbSeg];
-  if (blockIdx.x == 0) {
-    if (tid < kFbSeg) fbn_next[tid * kFbLine] = 0u;
-    if (tid == 0) fbn[kFbSeg * kFbLine] = total;
-  }
+  if (blockIdx.x == 0 && tid == 0) fbn[kFbSeg * kFbLine] = total;
   for (uint32_t i = blockIdx.x * kBlock + tid; i < total; i += gridDim.x * kBlock) {
     int k = 0;  // s_pre[k] <= i < s_pre[k + 1]
 #pragma unroll
@@ -598,6 +599,10 @@ k_shade_list(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, cons
     const uint32_t f = (uint32_t)keys[P] - a.mesh.id_base;
     const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
     store_rgb(img, P, shade_pixel<PROG, IBL>(a, f, px, py));
+  }
+  if (tid == 0 && atomicAdd(fbn + (kFbSeg + 1) * kFbLine, 1u) == gridDim.x - 1) {
+    for (int k = 0; k < kFbSeg; k++) fbn[k * kFbLine] = 0u;
+    fbn[(kFbSeg + 1) * kFbLine] = 0u;
   }
 }
 

@@ -47,11 +47,10 @@ struct tina_ctx {
   uint32_t* spans = nullptr;      // per-face tile spans (k_setup -> k_bin)
   int ncu = 0;                    // compute units of the device
   // learned-shading fallback queue (kFbSeg segments of fb_segcap pixel indices),
-  // two buffers of segment counters, and k_shade_nn's resident grid
+  // its segment counters, and k_shade_nn's resident grid
   uint32_t* fbq = nullptr;
   uint32_t* fbn = nullptr;
   long fb_segcap = 0;
-  int fb_which = 0;
   int nn_grid = 0;
   // tina_clear only records the clear; the next call that touches the keys applies it
   // (the binned raster by starting its tiles from empty keys, anything else with
@@ -356,13 +355,13 @@ int tina_ctx_create(int width, int height, int max_faces, tina_ctx** out) {
   if (hipMalloc(&c->keys, sizeof(uint64_t) * (size_t)width * height) != hipSuccess ||
       hipMalloc(&c->ctr, sizeof(Counters)) != hipSuccess || hipMalloc(&c->cursor, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->fbq, sizeof(uint32_t) * (size_t)c->fb_segcap * kFbSeg) != hipSuccess ||
-      hipMalloc(&c->fbn, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess) {
+      hipMalloc(&c->fbn, sizeof(uint32_t) * kFbWords) != hipSuccess) {
     (void)hipGetLastError();
     release();
     return fail(nullptr, TINA_ERR_ALLOC, "hipMalloc of the depth|id buffer failed");
   }
   if (hipMemset(c->ctr, 0, sizeof(Counters)) != hipSuccess || hipMemset(c->cursor, 0, sizeof(uint32_t)) != hipSuccess ||
-      hipMemset(c->fbn, 0, 2 * sizeof(uint32_t) * kFbWords) != hipSuccess ||
+      hipMemset(c->fbn, 0, sizeof(uint32_t) * kFbWords) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     release();
     return fail(nullptr, TINA_ERR_HIP, "counter initialisation failed");
@@ -938,23 +937,20 @@ int tina_shade_nn(tina_ctx* c, const tina_mesh* mesh, const tina_material* mat, 
   if (rc) return rc;
   if ((rc = flush_clear(c, (hipStream_t)stream))) return rc;
   const long npix = (long)c->W * c->H;
-  const int which = c->fb_which;
-  c->fb_which ^= 1;
   hipStream_t s = (hipStream_t)stream;
-  uint32_t* fbn = c->fbn + which * kFbWords;
-  uint32_t* fbn_next = c->fbn + (which ^ 1) * kFbWords;
+  uint32_t* fbn = c->fbn;
   hipLaunchKernelGGL(k_shade_nn, dim3(grid_for(npix, kNnShadePix, (unsigned)c->nn_grid)), dim3(kBlock), 0, s, a,
                      c->keys, (const __bf16*)d_packed, threshold, c->fbq, c->fb_segcap, fbn, d_img);
   TINA_LAUNCH(c, "k_shade_nn");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade_list<PROG_PBR, true>), dim3(grid_for(npix, kBlock, 512)), dim3(kBlock), 0,
-                     s, a, c->keys, c->fbq, c->fb_segcap, fbn, fbn_next, d_img);
+                     s, a, c->keys, c->fbq, c->fb_segcap, fbn, d_img);
   TINA_LAUNCH(c, "k_shade_list");
   return TINA_OK;
 }
 
 int tina_nn_fallback_count(tina_ctx* c, uint32_t* d_count, void* stream) {
   if (!c || !d_count) return fail(c, TINA_ERR_INVALID, "ctx or out is NULL");
-  TINA_HIP(c, hipMemcpyAsync(d_count, c->fbn + (c->fb_which ^ 1) * kFbWords + kFbSeg * kFbLine, sizeof(uint32_t),
+  TINA_HIP(c, hipMemcpyAsync(d_count, c->fbn + kFbSeg * kFbLine, sizeof(uint32_t),
                              hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return TINA_OK;
 }

@@ -52,3 +52,40 @@ def test_captured_raster_replays_like_eager(gpu, n):
         torch.cuda.synchronize()
         assert torch.equal(ids, ids0), f'replay {k}: {(ids != ids0).sum().item()} ids differ'
         assert torch.equal(d, d0), f'replay {k}'
+
+
+def test_captured_neural_frame_replays_like_eager(gpu):
+    """The surrogate's pass (k_shade_nn + the analytic fallback queue) captured
+    with the rest of Scene.render(): its queue counters are emptied by the pass
+    itself, so each replay shades the same pixels the eager frame does (an
+    untrained network, so that many pixels take the fallback)."""
+    import torch
+    from taichi_three_amd import workloads as wl
+    from taichi_three_amd.neural import NeuralShading
+    print('case: captured neural frame 320x180', flush=True)
+    w = wl.ibl_1080p(seed=6, res=(320, 180), nfaces=50_000, sky=(128, 64), precision=4)
+    scene, mesh = wl.build_scene(w)
+    ns = NeuralShading(scene.lighting, seed=3)
+    scene.neural = ns
+    scene.render()
+    torch.cuda.synchronize()
+    ref = scene.img.to_numpy().copy()
+    nref = ns.fallback_count(scene.engine)
+    assert 0 < nref < 320 * 180
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        scene.render()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        scene.render()
+    for k in range(3):
+        scene.image.storage.fill_(float(k + 1))
+        scene.render()
+        g.replay()
+        torch.cuda.synchronize()
+        img = scene.img.to_numpy()
+        assert np.array_equal(img, ref), f'replay {k}: {int((img != ref).any(axis=-1).sum())} pixels differ'
+        assert ns.fallback_count(scene.engine) == nref
