@@ -39,11 +39,13 @@ constexpr int kNnPackBytes = kNnFrags * 64 * 16 + (3 * 64 + 4) * 4;
 // Fallback queue of k_shade_nn: pixel chunk c appends to segment c % kFbSeg,
 // whose counter sits alone on a 128-byte line (one global atomic per chunk).
 // kFbSeg counters kFbLine words apart, then the last pass's total, then the
-// ticket k_shade_list's workgroups take when they finish: the last one empties
-// the counters and the ticket, so every call starts from zeros (a captured call
-// replays as it ran).
-constexpr int kFbSeg = 32, kFbLine = 32;
-constexpr int kFbWords = (kFbSeg + 2) * kFbLine;
+// tickets k_shade_list's workgroups take when they finish (kFbTick of them, one
+// per workgroup residue, and a top one): the last workgroup empties the counters
+// and the tickets, so every call starts from zeros (a captured call replays as
+// it ran).  Spreading the tickets keeps ~500 workgroups that find the queue
+// empty and finish together from serialising on one word.
+constexpr int kFbSeg = 32, kFbLine = 32, kFbTick = 16;
+constexpr int kFbWords = (kFbSeg + 2 + kFbTick) * kFbLine;
 
 // ---- random inputs -----------------------------------------------------------
 
@@ -600,9 +602,16 @@ k_shade_list(const ShadeArgs a_by_value, const uint64_t* __restrict__ keys, cons
     const int px = (int)(P % a.cam.W), py = (int)(P / a.cam.W);
     store_rgb(img, P, shade_pixel<PROG, IBL>(a, f, px, py));
   }
-  if (tid == 0 && atomicAdd(fbn + (kFbSeg + 1) * kFbLine, 1u) == gridDim.x - 1) {
-    for (int k = 0; k < kFbSeg; k++) fbn[k * kFbLine] = 0u;
-    fbn[(kFbSeg + 1) * kFbLine] = 0u;
+  if (tid == 0) {
+    const unsigned sub = blockIdx.x % kFbTick, nsub = min(gridDim.x, (unsigned)kFbTick);
+    uint32_t* const tick = fbn + (kFbSeg + 2 + sub) * kFbLine;
+    if (atomicAdd(tick, 1u) == (gridDim.x - sub + kFbTick - 1) / kFbTick - 1) {
+      *tick = 0u;  // every workgroup of this residue has taken its ticket
+      if (atomicAdd(fbn + (kFbSeg + 1) * kFbLine, 1u) == nsub - 1) {
+        for (int k = 0; k < kFbSeg; k++) fbn[k * kFbLine] = 0u;
+        fbn[(kFbSeg + 1) * kFbLine] = 0u;
+      }
+    }
   }
 }
 
