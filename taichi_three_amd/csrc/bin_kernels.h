@@ -96,8 +96,8 @@ struct FaceRec {
 // One raster call on the binned path: its faces, screen, key buffer and the
 // scratch the passes share.  bin_faces (faces per binning workgroup) and
 // max_tiles (bbox tiles above which a face is rasterised as a giant) are the
-// tunable bin sizes; a single raster call takes bin_faces_for(ntiles) and
-// kBinMaxTiles.
+// tunable bin sizes; a single raster call takes bin_faces_call(ntiles, T, CUs)
+// and kBinMaxTiles.
 struct BinJob {
   const float* verts;
   uint64_t* keys;
@@ -114,7 +114,7 @@ struct BinJob {
   BigRec* giantq;   // [T] the set-up of each giant face, at its face index
   uint32_t* gids;   // [T] giant face ids by group (see above)
   uint32_t* gcnt;   // [ngroups] giant faces of each binning group
-  uint32_t* spans;        // [T] per-face tile spans written by k_setup, or null: k_bin sets the faces up
+  uint32_t* spans;  // [T] per-face tile spans written by k_setup, or null: k_bin sets the faces up
   Cam cam;
   int T;
   uint32_t id_base, flags;
