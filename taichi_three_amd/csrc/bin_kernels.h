@@ -36,6 +36,7 @@ static_assert(6 + 2 * kTileLog <= 16, "fragment code does not fit 16 bits");
 static_assert(64 << (2 * kTileLog) <= (1 << 14), "fragment offset does not fit 14 bits");
 constexpr int kTile = 1 << kTileLog;  // 16 x 16 pixel tiles, 2 KB of keys in LDS
 constexpr int kBinBlock = 1024;       // lanes of a binning workgroup
+constexpr int kTileBlock = 256;       // lanes of a tile workgroup (128: k_tile 91 us, 512: 97 us against 71 at 1080p/2M)
 constexpr int kBinMaxTiles = 64;      // bbox tiles above which a face is rasterised as a giant
 // binning workgroups a k_tile workgroup can index: its g + 1 words of dynamic
 // LDS plus ~21 KB of static LDS stay within a 64 KB workgroup allocation
@@ -443,13 +444,13 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   // tags above, keep seven workgroups per CU up to 336 groups
   uint16_t* const gc = (uint16_t*)(pre + j.ngroups + 1);
   __shared__ unsigned long long tk[kTile * kTile];
-  __shared__ float4 R0[kBlock], R1[kBlock], R2[kBlock], R3[kBlock];
+  __shared__ float4 R0[kTileBlock], R1[kTileBlock], R2[kTileBlock], R3[kTileBlock];
   // per wave, per live face: its first fragment pair (bits 0-13) and ceil(2**16 / pairs a row) (bits 14-30)
-  __shared__ uint32_t Lstart[kBlock];
+  __shared__ uint32_t Lstart[kTileBlock];
   // per wave: window tag (a window start below 2**13) at the position of a face's last fragment pair
-  __shared__ int16_t Mark[kBlock];
-  __shared__ uint16_t Q[kBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
-  __shared__ uint32_t s_wsum[kBlock / 64], s_carry, s_next;
+  __shared__ int16_t Mark[kTileBlock];
+  __shared__ uint16_t Q[kTileBlock * 2];  // per wave: up to 127 pending fragments (face slot | lx | ly)
+  __shared__ uint32_t s_wsum[kTileBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
   const Cam& cam = j.cam;
   const int tiles_x = j.tiles_x, ngroups = j.ngroups;
@@ -464,7 +465,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   const int ntiles = j.ntiles;  // chunk tables are group-major: [group][tile]
   // (Staging the chunks' entry offsets in LDS with the counts, to take one
   // dependent read out of the face loop, measured no faster.)
-  const uint32_t total = block_exclusive_scan<kBlock>(
+  const uint32_t total = block_exclusive_scan<kTileBlock>(
       ngroups,
       [&](int c) {
         const uint32_t ng = j.gcnt[c];
@@ -475,7 +476,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   const int tw = min(kTile, cam.W - ox), th = min(kTile, cam.H - oy);
   if (total == 0) {
     if (!j.load_keys)  // the deferred clear of a tile no face touches
-      for (int i = tid; i < kTile * kTile; i += kBlock) {
+      for (int i = tid; i < kTile * kTile; i += kTileBlock) {
         const int lx = i & (kTile - 1), ly = i >> kTileLog;
         if (lx < tw && ly < th) keys[(size_t)(oy + ly) * cam.W + ox + lx] = empty_key();
       }
@@ -485,7 +486,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     pre[ngroups] = total;
     s_next = 0;
   }
-  for (int i = tid; i < kTile * kTile; i += kBlock) {
+  for (int i = tid; i < kTile * kTile; i += kTileBlock) {
     const int lx = i & (kTile - 1), ly = i >> kTileLog;
     tk[i] = (j.load_keys && lx < tw && ly < th) ? keys[(size_t)(oy + ly) * cam.W + ox + lx] : empty_key();
   }
@@ -648,7 +649,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
     wave_sync_lds();
   }
   __syncthreads();
-  for (int i = tid; i < kTile * kTile; i += kBlock) {
+  for (int i = tid; i < kTile * kTile; i += kTileBlock) {
     const int lx = i & (kTile - 1), ly = i >> kTileLog;
     if (lx < tw && ly < th) keys[(size_t)(oy + ly) * cam.W + ox + lx] = tk[i];
   }
@@ -701,7 +702,7 @@ __device__ __forceinline__ long strip_item(long b, long n) {
   return 32 * (8 * (b / 256) + b % 8) + (b / 8) % 32;
 }
 
-__global__ void __launch_bounds__(kBlock) k_tile(const BinJob jv) {
+__global__ void __launch_bounds__(kTileBlock) k_tile(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
   // the binning groups have all reserved their entries (stream order) and no tile
   // reads the cursor: empty it for the next call
@@ -734,11 +735,11 @@ k_bbin(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njob
   }
 }
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kTileBlock)
 k_btile(const BinJob* __restrict__ jobs, const int* __restrict__ prefix, int njobs) {
   // every scene's binning is done and no tile reads the cursors: empty them
   if (blockIdx.x == 0)
-    for (int s = threadIdx.x; s < njobs; s += kBlock) *jobs[s].cursor = 0u;
+    for (int s = threadIdx.x; s < njobs; s += kTileBlock) *jobs[s].cursor = 0u;
   const long total = prefix[njobs];
   for (long b = blockIdx.x; b < total; b += gridDim.x) {
     const long sweep = b - blockIdx.x, width = min((long)gridDim.x, total - sweep);

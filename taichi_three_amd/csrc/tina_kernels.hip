@@ -471,7 +471,7 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     }
     hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + j.bin_faces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
-    hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kBlock), tile_lds_bytes(j.ngroups), s, j);
+    hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kTileBlock), tile_lds_bytes(j.ngroups), s, j);
     TINA_LAUNCH(c, "k_tile");
     return TINA_OK;
   }
@@ -1205,7 +1205,7 @@ int tina_batch_render(tina_batch* b, void* stream) {
                      (size_t)(b->max_ntiles + b->tune.bin_faces) * 4, s, b->d_jobs, b->d_prefix, n);
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_bbin"))) return rc;
   // every tile of every scene starts from empty keys and is written whole: no clear pass
-  hipLaunchKernelGGL(k_btile, dim3(batch_grid(b->tune.grid_tile, b->totals[1])), dim3(kBlock),
+  hipLaunchKernelGGL(k_btile, dim3(batch_grid(b->tune.grid_tile, b->totals[1])), dim3(kTileBlock),
                      tile_lds_bytes(b->max_groups), s, b->d_jobs, b->d_prefix + (n + 1), n);
   if ((rc = hip_check(nullptr, hipGetLastError(), "k_btile"))) return rc;
   const unsigned gs = batch_grid(b->tune.grid_shade, b->totals[2]);
