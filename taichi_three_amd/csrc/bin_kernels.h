@@ -453,6 +453,9 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
   __shared__ uint32_t s_wsum[kTileBlock / 64], s_carry, s_next;
   const int tid = threadIdx.x, lane = __lane_id(), wbase = tid & ~63;
   const Cam& cam = j.cam;
+  // read once: the kernel argument block is not provably unchanged across the
+  // loops' stores, so uses of cam.bias inside them reload it
+  const float bias_x = cam.bias[0], bias_y = cam.bias[1];
   const int tiles_x = j.tiles_x, ngroups = j.ngroups;
   const uint32_t id_base = j.id_base;
   const uint32_t* __restrict__ chunk_off = j.chunk_off;
@@ -586,7 +589,7 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
       // the reference's weights and test as written: nearly every full wave of
       // survivors holds a covered pixel, so the divisions run anyway and the
       // sign-based early rejection of pixel_cover_fast would only add work
-      if (pixel_weights(s, (float)(ox + lx) + cam.bias[0], (float)(oy + ly) + cam.bias[1], w0, w1, w2)) {
+      if (pixel_weights(s, (float)(ox + lx) + bias_x, (float)(oy + ly) + bias_y, w0, w1, w2)) {
         const uint64_t key = pack_key(pixel_depth(s, w0, w1, w2), (uint32_t)__float_as_int(r3.w));
         atomicMin(&tk[(ly << kTileLog) + lx], (unsigned long long)key);
       }
@@ -618,9 +621,9 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
         s.bx = r0.x; s.by = r0.y; s.cx = r0.z; s.cy = r0.w;
         s.bcnx = r1.x; s.bcny = r1.y; s.canx = r1.z; s.cany = r1.w;
         const bool wellc = (packed >> 24) != 0;
-        const float posy = (float)(oy + ly) + cam.bias[1];
-        keep0 = !(wellc && pixel_rejects(s, (float)(ox + lx) + cam.bias[0], posy));
-        keep1 = dx + 1 < wj && !(wellc && pixel_rejects(s, (float)(ox + lx + 1) + cam.bias[0], posy));
+        const float posy = (float)(oy + ly) + bias_y;
+        keep0 = !(wellc && pixel_rejects(s, (float)(ox + lx) + bias_x, posy));
+        keep1 = dx + 1 < wj && !(wellc && pixel_rejects(s, (float)(ox + lx + 1) + bias_x, posy));
         code = (k << (2 * kTileLog)) | (ly << kTileLog) | lx;
       }
       kbase += __popcll(ends);
