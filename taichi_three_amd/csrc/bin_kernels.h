@@ -38,9 +38,10 @@ constexpr int kTile = 1 << kTileLog;  // 16 x 16 pixel tiles, 2 KB of keys in LD
 constexpr int kBinBlock = 1024;       // lanes of a binning workgroup
 constexpr int kTileBlock = 256;       // lanes of a tile workgroup (128: k_tile 91 us, 512: 97 us against 71 at 1080p/2M)
 constexpr int kBinMaxTiles = 64;      // bbox tiles above which a face is rasterised as a giant
-// binning workgroups a k_tile workgroup can index: its g + 1 words of dynamic
-// LDS plus ~21 KB of static LDS stay within a 64 KB workgroup allocation
-constexpr int kMaxBinGroups = 8192;
+// binning workgroups a k_tile workgroup can index: its dynamic LDS of g + 1
+// words and g half-words (tile_lds_bytes) plus ~21 KB of static LDS stay within
+// a 64 KB workgroup allocation (7,168 groups: 64,238 bytes)
+constexpr int kMaxBinGroups = 7168;
 // bytes of dynamic LDS a binning workgroup may take (its static LDS is 72 bytes):
 // a gfx950 CU has 160 KB of LDS and one workgroup may use all of it, so screens
 // up to ~40k tiles (3840x2160 is 32,400) stay on the binned path
@@ -55,38 +56,37 @@ inline int bin_faces_for(long ntiles) {
   return 0;
 }
 
-// Faces per binning group of a single raster call of nfaces faces on ncu CUs.
-// A call of 256 faces a CU or more is cut into equal groups, a whole number of
-// rounds of one group per CU, so that no CU waits while others finish a
-// partial last round and the faces are set up inside the binning pass (at
-// 1080p: 2 M faces take 256 groups of 7,813 faces instead of 245 of 8,192;
-// 1.1 M, 256 of 4,297 instead of 135 of 8,192; 70 k to 900 k faces, 256 groups
-// instead of 9 to 110 groups of 8,192 after a separate set-up pass, 1.5-4 us
-// faster for the two raster kernels).  Smaller calls keep the largest group,
-// their faces set up in a pass of their own.
-constexpr long kBinFacesPerCu = 256;
+// Faces per binning group of a single raster call of nfaces faces on ncu CUs:
+// equal groups, a whole number of rounds of one group per CU, so that no CU
+// waits while others finish a partial last round, and at least kBinMinFaces
+// faces each, since a group clears, scans and writes the whole tile table
+// whatever its face count.  Every group sets its own faces up.  At 1080p: 2 M
+// faces take 256 groups of 7,813 faces instead of 245 of 8,192; 1.1 M, 256 of
+// 4,297 instead of 135 of 8,192; 70 k to 900 k faces, 256 groups instead of 9
+// to 110 groups of 8,192 after a separate set-up pass of one face a lane
+// (1.5-4 us faster for the two raster kernels); 1 k to 20 k faces, 32 to 254
+// groups of 32 to 79 faces instead of one group of 8,192 after that pass
+// (2.3-3.1 us faster).
+constexpr int kBinMinFaces = 32;
 inline int bin_faces_call(long ntiles, long nfaces, int ncu) {
   const int bf = bin_faces_for(ntiles);
   if (bf == 0 || ncu <= 0) return bf;
-  const long groups = (nfaces + bf - 1) / bf;
-  if (nfaces < (long)ncu * kBinFacesPerCu || groups % ncu == 0) return bf;
   const long rounds = (nfaces + (long)ncu * bf - 1) / ((long)ncu * bf);
-  const long eq = (nfaces + rounds * ncu - 1) / (rounds * ncu);
-  return (int)(eq < bf ? eq : bf);
+  const long eq = rounds > 0 ? (nfaces + rounds * ncu - 1) / (rounds * ncu) : bf;
+  return (int)(eq < kBinMinFaces ? kBinMinFaces : eq < bf ? eq : bf);
 }
 
 // Binning groups to allocate for calls of up to cap faces: the most any call
-// of at most cap faces can use under bin_faces_call.
+// of at most cap faces can use under bin_faces_call (both bounds below grow
+// with the face count).
 inline long bin_groups_cap(long ntiles, long cap, int ncu) {
   const int bf = bin_faces_for(ntiles);
   if (bf == 0) return 0;
-  const long groups = (cap + bf - 1) / bf;
-  if (ncu <= 0 || cap < (long)ncu * kBinFacesPerCu) return groups;
-  return (cap + (long)ncu * bf - 1) / ((long)ncu * bf) * ncu;
+  if (ncu <= 0) return (cap + bf - 1) / bf;
+  const long rounds = (cap + (long)ncu * bf - 1) / ((long)ncu * bf);
+  const long small = (cap + kBinMinFaces - 1) / kBinMinFaces;
+  return rounds * ncu < small ? rounds * ncu : small;
 }
-
-// A face span value k_setup writes for a giant face (its set-up is in giantq).
-constexpr uint32_t kSpanGiant = 0xFFFFFFFEu;
 
 // Setup record of a binned face (64 B, one cache line): the state the coverage
 // loop needs (triangle.py:127-131 plus the view-space z of A, B, C) and its bbox.
@@ -118,7 +118,6 @@ struct BinJob {
   BigRec* giantq;   // [T] the set-up of each giant face, at its face index
   uint32_t* gids;   // [T] giant face ids by group (see above)
   uint32_t* gcnt;   // [ngroups] giant faces of each binning group
-  uint32_t* spans;  // [T] per-face tile spans written by k_setup, or null: k_bin sets the faces up
   Cam cam;
   int T;
   uint32_t id_base, flags;
@@ -359,50 +358,36 @@ __device__ __forceinline__ void bin_group(const BinJob& j, int g) {
   if (tid == 0) s_gcnt = 0;
   __syncthreads();
 
-  if (j.spans) {
-    // the faces were set up by k_setup: read their spans, list the giants
-    for (int i = tid; i < nbf; i += kBinBlock) {
-      const long f = f0 + i;
-      uint32_t sp = f < j.T ? j.spans[f] : 0xFFFFFFFFu;
-      if (sp == kSpanGiant) {
-        gids[atomicAdd(&s_gcnt, 1u)] = (uint32_t)f;
-        sp = 0xFFFFFFFFu;
-      }
-      span[i] = sp;
-      count_span(cnt, tiles_x, sp);
+  // The positions of a lane's faces are fetched two faces ahead of their set-up,
+  // so that the loads overlap the arithmetic of the faces before them.
+  const float* __restrict__ verts = j.verts;
+  auto fetch = [&](int i, float (&v)[9]) {
+    const long f = f0 + i;
+    if (i < nbf && f < j.T) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) v[k] = verts[f * 9 + k];
     }
-  } else {
-    // The positions of a lane's faces are fetched two faces ahead of their set-up,
-    // so that the loads overlap the arithmetic of the faces before them.
-    const float* __restrict__ verts = j.verts;
-    auto fetch = [&](int i, float (&v)[9]) {
-      const long f = f0 + i;
-      if (i < nbf && f < j.T) {
+  };
+  float p0[9], p1[9];
+  fetch(tid, p0);
+  fetch(tid + kBinBlock, p1);
+  for (int i = tid; i < nbf; i += kBinBlock) {
+    const long f = f0 + i;
+    float v[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) v[k] = verts[f * 9 + k];
-      }
-    };
-    float p0[9], p1[9];
-    fetch(tid, p0);
-    fetch(tid + kBinBlock, p1);
-    for (int i = tid; i < nbf; i += kBinBlock) {
-      const long f = f0 + i;
-      float v[9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) {
-        v[k] = p0[k];
-        p0[k] = p1[k];
-      }
-      fetch(i + 2 * kBinBlock, p1);
-      FaceSetup s;
-      bool giant;
-      const uint32_t sp = face_setup_record(j, f, v, s, giant);
-      count_span(cnt, tiles_x, sp);
-      span[i] = sp;
-      if (giant) {
-        store_giant(j, f, s);
-        gids[atomicAdd(&s_gcnt, 1u)] = (uint32_t)f;
-      }
+    for (int k = 0; k < 9; k++) {
+      v[k] = p0[k];
+      p0[k] = p1[k];
+    }
+    fetch(i + 2 * kBinBlock, p1);
+    FaceSetup s;
+    bool giant;
+    const uint32_t sp = face_setup_record(j, f, v, s, giant);
+    count_span(cnt, tiles_x, sp);
+    span[i] = sp;
+    if (giant) {
+      store_giant(j, f, s);
+      gids[atomicAdd(&s_gcnt, 1u)] = (uint32_t)f;
     }
   }
   __syncthreads();
@@ -673,25 +658,8 @@ __device__ __forceinline__ void tile_body(const BinJob& j, int t) {
 #define TINA_KERNARG_JOB(T, name, byval) const T& name = byval
 #endif
 
-// Set-up of every face of the call, one face per lane: setup records, tile
-// spans (kSpanGiant for a giant face, whose set-up goes to giantq) for k_bin.
-__global__ void __launch_bounds__(kBlock) k_setup(const BinJob jv) {
-  TINA_KERNARG_JOB(BinJob, j, jv);
-  const long f = (long)blockIdx.x * kBlock + threadIdx.x;
-  float v[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) v[k] = f < j.T ? j.verts[f * 9 + k] : 0.0f;
-  FaceSetup s;
-  bool giant;
-  const uint32_t sp = face_setup_record(j, f, v, s, giant);
-  if (giant) store_giant(j, f, s);
-  if (f < j.T) j.spans[f] = giant ? kSpanGiant : sp;
-}
-
 __global__ void __launch_bounds__(kBinBlock) k_bin(const BinJob jv) {
   TINA_KERNARG_JOB(BinJob, j, jv);
-  // without a set-up pass this is the call's first kernel: it empties the next
-  // call's counters (see k_setup)
   bin_group(j, blockIdx.x);
 }
 

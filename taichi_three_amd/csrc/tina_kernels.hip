@@ -44,7 +44,6 @@ struct tina_ctx {
   uint32_t* gcnt = nullptr;
   uint32_t* cursor = nullptr;     // the binning groups' entry cursor (zero between calls)
   FaceRec* recs = nullptr;
-  uint32_t* spans = nullptr;      // per-face tile spans (k_setup -> k_bin)
   int ncu = 0;                    // compute units of the device
   // learned-shading fallback queue (kFbSeg segments of fb_segcap pixel indices),
   // its segment counters, and k_shade_nn's resident grid
@@ -144,12 +143,11 @@ int flush_clear(tina_ctx* c, hipStream_t stream) {
 
 void free_scratch(tina_ctx* c) {
   for (void* p : {(void*)c->bigq, (void*)c->giantq, (void*)c->chunk_off, (void*)c->chunk_cnt, (void*)c->entries,
-                  (void*)c->gids, (void*)c->gcnt, (void*)c->recs, (void*)c->spans})
+                  (void*)c->gids, (void*)c->gcnt, (void*)c->recs})
     if (p) (void)hipFree(p);
   c->bigq = c->giantq = nullptr;
   c->chunk_off = c->chunk_cnt = c->entries = c->gids = c->gcnt = nullptr;
   c->recs = nullptr;
-  c->spans = nullptr;
   c->groups_cap = 0;
 }
 
@@ -178,8 +176,7 @@ int alloc_scratch(tina_ctx* c, int cap) {
          hipMalloc(&c->entries, 4 * n * kBinMaxTiles) == hipSuccess &&
          hipMalloc(&c->gids, sizeof(uint32_t) * n) == hipSuccess &&
          hipMalloc(&c->gcnt, sizeof(uint32_t) * groups) == hipSuccess &&
-         hipMalloc(&c->recs, sizeof(FaceRec) * n) == hipSuccess &&
-         hipMalloc(&c->spans, sizeof(uint32_t) * n) == hipSuccess;
+         hipMalloc(&c->recs, sizeof(FaceRec) * n) == hipSuccess;
     if (ok) c->groups_cap = (int)groups;
   }
   if (!ok) {
@@ -454,22 +451,11 @@ int tina_raster(tina_ctx* c, const float* d_verts, int nfaces, uint32_t id_base,
     j.bin_faces = bin_faces_call(j.ntiles, nfaces, c->ncu);
     j.ngroups = (nfaces + j.bin_faces - 1) / j.bin_faces;
     j.max_tiles = kBinMaxTiles;
-    // Few binning groups leave most CUs idle while each group sets its faces up
-    // one after another (1080p, 20 k faces: 3 groups, k_bin 45 us); a set-up pass
-    // over all faces first then takes that work off the groups (5 + 18 us).  Calls
-    // of 256 faces a CU or more get a group per CU (bin_faces_call), and then the
-    // single pass is faster (200 k faces: 44.7 us for both raster kernels against
-    // 48.5 us with the set-up pass; 2 M faces: 61 against 39 + 26 us).
-    j.spans = (long)j.ngroups * 2 < (long)c->ncu ? c->spans : nullptr;
     // a clear recorded by tina_clear is applied by k_tile itself: it starts every
     // tile from empty keys and writes all of them
     j.load_keys = c->clear_pending ? 0 : 1;
     c->clear_pending = false;
     allow_big_lds();
-    if (j.spans) {
-      hipLaunchKernelGGL(k_setup, dim3((unsigned)((nfaces + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, j);
-      TINA_LAUNCH(c, "k_setup");
-    }
     hipLaunchKernelGGL(k_bin, dim3(j.ngroups), dim3(kBinBlock), (size_t)(j.ntiles + j.bin_faces) * 4, s, j);
     TINA_LAUNCH(c, "k_bin");
     hipLaunchKernelGGL(k_tile, dim3(j.ntiles), dim3(kTileBlock), tile_lds_bytes(j.ngroups), s, j);
@@ -1129,7 +1115,6 @@ int tina_batch_create(int nscenes, const tina_scene_desc* scenes, const tina_bat
     j.giantq = (BigRec*)gq;
     j.gids = (uint32_t*)gi;
     j.gcnt = (uint32_t*)gn;
-    j.spans = nullptr;  // k_bbin sets its faces up itself
     memcpy(j.cam.W2V, d.cam.W2V, sizeof(j.cam.W2V));
     memcpy(j.cam.V2W, d.cam.V2W, sizeof(j.cam.V2W));
     j.cam.bias[0] = d.cam.bias[0];
