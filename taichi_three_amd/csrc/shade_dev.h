@@ -23,6 +23,13 @@ __device__ __forceinline__ V3 vdivs(V3 a, float s) {
   return v3(div_rn(a.x, R), div_rn(a.y, R), div_rn(a.z, R));
 }
 __device__ __forceinline__ V3 vneg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+// a / d as a times the hardware reciprocal of d (v_rcp_f32, within 1 ulp), for
+// the quotients that only scale a light's contribution -- its 1/d^2 falloff and
+// the Cook-Torrance D and G terms: the colour moves by a few ulps of its own
+// value.  The normalisations, the view ray, the interpolation weights and ACES
+// keep the correctly rounded division (post passes such as FXAA branch on the
+// tone-mapped values, so those stay the binary32 oracle's to the bit).
+__device__ __forceinline__ float div_scale(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
 __device__ __forceinline__ float vdot(V3 a, V3 b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
 // Matrix.normalized(): v * (1 / sqrt(sum v^2))
 __device__ __forceinline__ V3 vnormalized(V3 a) { const float inv = fdiv_rn(1.0f, sqrtf(vdot(a, a))); return vsmul(inv, a); }
@@ -103,11 +110,11 @@ __device__ __forceinline__ V3 cooktorrance(float roughness, V3 f0, V3 n, V3 L, V
   const float VoH = fminf(ONE_M_EPS, fmaxf(EPS, vdot(half, V)));
   const float alpha2 = fmaxf(1e-6f, roughness * roughness);
   const float denom = 1.0f - (NoH * NoH) * (1.0f - alpha2);
-  const float ndf = fdiv_rn(alpha2, denom * denom);
-  const float k = fdiv_rn(alpha2, 2.0f);
-  float vdf = fdiv_rn(1.0f, ((NoV * k) + 1.0f) - k);
-  vdf = vdf * fdiv_rn(1.0f, ((NoL * k) + 1.0f) - k);
-  vdf = fdiv_rn(vdf, (1.0f * (1.0f - alpha2)) + ((float)(4.0 * 3.141592653589793) * alpha2));
+  const float ndf = div_scale(alpha2, denom * denom);
+  const float k = alpha2 * 0.5f;  // exactly alpha2 / 2 (alpha2 >= 1e-6)
+  float vdf = div_scale(1.0f, ((NoV * k) + 1.0f) - k);
+  vdf = vdf * div_scale(1.0f, ((NoL * k) + 1.0f) - k);
+  vdf = div_scale(vdf, (1.0f * (1.0f - alpha2)) + ((float)(4.0 * 3.141592653589793) * alpha2));
   // (1 - VoH)**5 as ((x * x) * (x * x)) * x: the integer power restated by
   // squaring, as the binary32 oracle does
   const float x1 = 1.0f - VoH, x2 = x1 * x1;
@@ -257,7 +264,7 @@ __device__ __forceinline__ V3 lighting(const tina_material& m, const Slots& pv, 
     const float cos_i = vdot(n, ldir);
     if (cos_i > 0.0f) {
       V3 lcolor = li.color;
-      lcolor = vdivs(lcolor, dist * dist);
+      lcolor = vsmul(div_scale(1.0f, dist * dist), lcolor);
       const V3 mcolor = run_material(m, pv, M_BRDF, n, ldir, V, ibl);
       res = vadd(res, vmul(vsmul(cos_i, lcolor), mcolor));
     }
@@ -578,7 +585,7 @@ __device__ __forceinline__ V3 shade_core(const ShadeArgs& a, V3 n, V3 pos, V3 V,
     const float cos_i = vdot(n, ldir);
     if (cos_i > 0.0f) {
       V3 lcolor = li.color;
-      lcolor = vdivs(lcolor, dist * dist);
+      lcolor = vsmul(div_scale(1.0f, dist * dist), lcolor);
       V3 mcolor;
       if constexpr (PROG == PROG_DIFFUSE) {
         mcolor = diff_brdf;
