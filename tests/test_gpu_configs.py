@@ -52,6 +52,7 @@ def _check_colour(img, o32, o64, tag):
     pixels, where no result within 1e-5 of it can meet both bounds; those pixels
     stay within 1e-5 of the binary32 value and are counted."""
     e32 = float(np.abs(img - o32).max())
+    print(f'{tag}: colour max |gpu - binary32| = {e32:.2e}', flush=True)
     assert e32 <= COLOR_TOL_F32, (tag, e32)
     d64 = np.abs(img - o64).max(axis=2)
     ref_ok = np.abs(o32 - o64).max(axis=2) <= COLOR_TOL_F64

@@ -47,6 +47,8 @@ def test_plain_512_bit_exact(gpu):
     assert (ids >= 0).sum() > 50000
     assert np.array_equal(ids, oids), f'{(ids != oids).sum()} id mismatches'
     assert np.array_equal(depth, odepth)
+    print(f'colour max |gpu - binary32| = {np.abs(img - o32).max():.2e}, '
+          f'|gpu - binary64| = {np.abs(img - o64).max():.2e}', flush=True)
     assert np.abs(img - o32).max() <= COLOR_TOL_F32
     assert np.abs(img - o64).max() <= COLOR_TOL_F64
 
@@ -75,6 +77,8 @@ def test_soup_small_pbr(gpu, seed):
     oids, odepth, o32, o64 = _oracle(w)
     assert np.array_equal(ids, oids), f'{(ids != oids).sum()} id mismatches'
     assert np.array_equal(depth, odepth)
+    print(f'colour max |gpu - binary32| = {np.abs(img - o32).max():.2e}, '
+          f'|gpu - binary64| = {np.abs(img - o64).max():.2e}', flush=True)
     assert np.abs(img - o32).max() <= COLOR_TOL_F32
     assert np.abs(img - o64).max() <= COLOR_TOL_F64
 
