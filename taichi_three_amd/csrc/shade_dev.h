@@ -89,14 +89,14 @@ __device__ __forceinline__ V3 skybox_sample(gf32* __restrict__ img, int w, int h
 __device__ __forceinline__ V3 phong(float m, V3 n, V3 L, V3 V) {
   const V3 rdir = vreflect(vneg(L), n);
   const float VoR = fmaxf(0.0f, vdot(V, rdir));
-  return vs(fdiv_rn(powf(VoR, m) * (m + 2.0f), 2.0f));
+  return vs((powf(VoR, m) * (m + 2.0f)) * 0.5f);  // x * 0.5 is x / 2 to the bit
 }
 
 // BlinnPhong (extension): max(0, N.normalize(L + V))**m * (m + 8) / 8
 __device__ __forceinline__ V3 blinnphong(float m, V3 n, V3 L, V3 V) {
   const V3 h = vnormalized(vadd(L, V));
   const float NoH = fmaxf(0.0f, vdot(n, h));
-  return vs(fdiv_rn(powf(NoH, m) * (m + 8.0f), 8.0f));
+  return vs((powf(NoH, m) * (m + 8.0f)) * 0.125f);  // x * 0.125 is x / 8 to the bit
 }
 
 // CookTorrance.sub_brdf / brdf (material.py:322-362)
@@ -111,7 +111,7 @@ __device__ __forceinline__ V3 cooktorrance(float roughness, V3 f0, V3 n, V3 L, V
   const float alpha2 = fmaxf(1e-6f, roughness * roughness);
   const float denom = 1.0f - (NoH * NoH) * (1.0f - alpha2);
   const float ndf = div_scale(alpha2, denom * denom);
-  const float k = alpha2 * 0.5f;  // exactly alpha2 / 2 (alpha2 >= 1e-6)
+  const float k = alpha2 * 0.5f;  // x * 0.5 is x / 2 to the bit
   float vdf = div_scale(1.0f, ((NoV * k) + 1.0f) - k);
   vdf = vdf * div_scale(1.0f, ((NoL * k) + 1.0f) - k);
   vdf = div_scale(vdf, (1.0f * (1.0f - alpha2)) + ((float)(4.0 * 3.141592653589793) * alpha2));

@@ -71,7 +71,7 @@ __device__ __forceinline__ void ct_sub(float roughness, V3 f0, V3 n, V3 L, V3 V,
   const float NoV = fmaxf(EPS, vdot(V, n));
   const float VoH = fminf(ONE_M_EPS, fmaxf(EPS, vdot(half, V)));
   const float alpha2 = fmaxf(1e-6f, roughness * roughness);
-  const float k = fdiv_rn(alpha2, 2.0f);
+  const float k = alpha2 * 0.5f;  // x * 0.5 is x / 2 to the bit
   vdf = fdiv_rn(1.0f, ((NoV * k) + 1.0f) - k);
   vdf = vdf * fdiv_rn(1.0f, ((NoL * k) + 1.0f) - k);
   vdf = fdiv_rn(vdf, (1.0f * (1.0f - alpha2)) + ((float)(4.0 * 3.141592653589793) * alpha2));
