@@ -174,8 +174,8 @@ int tina_set_camera(tina_ctx *ctx, const tina_camera *cam);
 /* Engine.clear_depth (engine.py:67-70) + TriangleRaster.occup = -1 (triangle.py:90-91):
  * every packed key becomes (depth 2**30, no face).  The clear is recorded and
  * applied by the next call that touches the keys, on that call's stream (the
- * binned tina_raster folds it into its binning kernel); tina_keys applies it on
- * this call's stream. */
+ * binned tina_raster folds it into its tile pass, which starts every tile from
+ * empty keys and writes it whole); tina_keys applies it on this call's stream. */
 int tina_clear(tina_ctx *ctx, void *stream);
 
 /* TriangleRaster.render_occup (triangle.py:88-131): setup, cull, clip, coverage,
