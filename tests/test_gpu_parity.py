@@ -416,9 +416,8 @@ def test_raster_random_scenes(gpu, seed):
 
 @pytest.mark.parametrize('seed', [0, 1, 2, 3])
 def test_raster_random_large_calls(gpu, seed):
-    """Calls of 1.1-1.5 M faces (and 3 M: two rounds of binning groups): a
-    binning group per CU or more, so k_bin sets the faces up itself (the
-    smaller calls above take the separate set-up pass).
+    """Calls of 1.1-1.5 M faces (and 3 M: two rounds of binning groups, one
+    group per CU in each).
     Random screens, cameras (some inside the cloud, so the clip rule acts), face
     sizes from sub-pixel to giant; ids and depth bit for bit against the
     oracle's parallel raster, culling on and off."""
@@ -456,3 +455,43 @@ def test_raster_random_large_calls(gpu, seed):
         assert (oi >= 0).sum() > 0.1 * W * H
         assert np.array_equal(ids.cpu().numpy(), oi), f'{(ids.cpu().numpy() != oi).sum()} id mismatches'
         assert np.array_equal(depth.cpu().numpy(), od)
+
+
+def test_raster_binning_group_boundaries(gpu):
+    """Face counts at the edges of the binning-group sizes (groups of at least
+    32 faces, whole rounds of one group per CU, at most 8,192 faces a group at
+    this screen): 1, 31-33, 255-257 faces, one group per CU and one face more or
+    less, and around a full round of 8,192-face groups.  Every call renders the
+    same scene's first n faces; ids and depth bit for bit against the oracle."""
+    import os
+    import torch
+    import taichi_three_amd as tina
+    from taichi_three_amd import workloads as wl
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    W, H = 640, 360
+    counts = sorted({1, 31, 32, 33, 255, 256, 257, 32 * ncu - 1, 32 * ncu, 32 * ncu + 1,
+                     8192 * ncu - 1, 8192 * ncu, 8192 * ncu + 1})
+    n = counts[-1]
+    rng = np.random.default_rng(77)
+    verts = wl.triangle_soup(rng, n, np.log(0.01), 0.7)
+    view = tina.lookat(back=(0.3, 0.4, 3.0))
+    proj = tina.perspective(60, W / H, 0.05, 500)
+    W2V, V2W = tina.camera_matrices(view, proj)
+    cam = orc.camera(W2V, V2W)
+    nthreads = max(1, len(os.sched_getaffinity(0)))
+    eng = tina.Engine(W, H, maxfaces=n)
+    eng.set_camera(view, proj)
+    r = tina.TriangleRaster(eng, maxfaces=n)
+    for k in counts:
+        print(f'case: binning boundary {W}x{H} n={k} ({ncu} CUs)', flush=True)
+        m = tina.SimpleMesh(maxfaces=k)
+        m.set_face_verts(verts[:k])
+        eng.clear_depth()
+        r.set_object(m)
+        r.render_occup()
+        ids, depth = eng.resolve()
+        torch.cuda.synchronize()
+        od, oi = orc.unpack(orc.raster_omp(W, H, cam, verts[:k], flags=3, nthreads=nthreads))
+        assert k < 255 or (oi >= 0).sum() > 20  # faces of ~0.5 px: 255 of them cover ~70 pixels
+        assert np.array_equal(ids.cpu().numpy(), oi), f'n={k}: {(ids.cpu().numpy() != oi).sum()} id mismatches'
+        assert np.array_equal(depth.cpu().numpy(), od), f'n={k}'
