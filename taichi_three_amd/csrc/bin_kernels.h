@@ -59,15 +59,17 @@ inline int bin_faces_for(long ntiles) {
 // Faces per binning group of a single raster call of nfaces faces on ncu CUs:
 // equal groups, a whole number of rounds of one group per CU, so that no CU
 // waits while others finish a partial last round, and at least kBinMinFaces
-// faces each, since a group clears, scans and writes the whole tile table
-// whatever its face count.  Every group sets its own faces up.  At 1080p: 2 M
-// faces take 256 groups of 7,813 faces instead of 245 of 8,192; 1.1 M, 256 of
-// 4,297 instead of 135 of 8,192; 70 k to 900 k faces, 256 groups instead of 9
-// to 110 groups of 8,192 after a separate set-up pass of one face a lane
-// (1.5-4 us faster for the two raster kernels); 1 k to 20 k faces, 32 to 254
-// groups of 32 to 79 faces instead of one group of 8,192 after that pass
-// (2.3-3.1 us faster).
-constexpr int kBinMinFaces = 32;
+// faces each: a group of up to 1,024 faces sets them up in one pass of its
+// lanes, and every group clears, scans and writes the whole tile table whatever
+// its face count.  At 1080p: 2 M faces take 256 groups of 7,813 faces instead
+// of 245 of 8,192; 1.1 M, 256 of 4,297 instead of 135 of 8,192; 70 k to 900 k
+// faces, 256 groups instead of 9 to 110 groups of 8,192 after a separate set-up
+// pass of one face a lane (1.5-4 us faster for the two raster kernels).
+// Minimum group sizes at 1080p, raster kernels for 1 k / 4 k / 20 k / 65 k /
+// 200 k faces: 32 faces 25.4 / 28.7 / 34.1 / 36.3 / 44.7 us, 512 faces
+// 24.9 / 27.2 / 29.8 / 33.3 / 44.7, 1,024 faces - / 27.9 / 30.8 / 32.9 / 43.6,
+// 2,048 faces (two set-up passes a lane) 32.7 / 35.5 / 37.5 / 45.1.
+constexpr int kBinMinFaces = 512;
 inline int bin_faces_call(long ntiles, long nfaces, int ncu) {
   const int bf = bin_faces_for(ntiles);
   if (bf == 0 || ncu <= 0) return bf;

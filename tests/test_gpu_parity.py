@@ -459,9 +459,9 @@ def test_raster_random_large_calls(gpu, seed):
 
 def test_raster_binning_group_boundaries(gpu):
     """Face counts at the edges of the binning-group sizes (groups of at least
-    32 faces, whole rounds of one group per CU, at most 8,192 faces a group at
-    this screen): 1, 31-33, 255-257 faces, one group per CU and one face more or
-    less, and around a full round of 8,192-face groups.  Every call renders the
+    512 faces, whole rounds of one group per CU, at most 8,192 faces a group at
+    this screen): 1, 31-33, 255-257 and 511-513 faces, 32 and 512 faces a CU
+    and one face more or less, and around a full round of 8,192-face groups.  Every call renders the
     same scene's first n faces; ids and depth bit for bit against the oracle."""
     import os
     import torch
@@ -469,8 +469,8 @@ def test_raster_binning_group_boundaries(gpu):
     from taichi_three_amd import workloads as wl
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
     W, H = 640, 360
-    counts = sorted({1, 31, 32, 33, 255, 256, 257, 32 * ncu - 1, 32 * ncu, 32 * ncu + 1,
-                     8192 * ncu - 1, 8192 * ncu, 8192 * ncu + 1})
+    counts = sorted({1, 31, 32, 33, 255, 256, 257, 511, 512, 513, 32 * ncu - 1, 32 * ncu, 32 * ncu + 1,
+                     512 * ncu - 1, 512 * ncu, 512 * ncu + 1, 8192 * ncu - 1, 8192 * ncu, 8192 * ncu + 1})
     n = counts[-1]
     rng = np.random.default_rng(77)
     verts = wl.triangle_soup(rng, n, np.log(0.01), 0.7)
